@@ -1,0 +1,154 @@
+"""Benchmark: seconds per full single-linkage tree of 1,000,000 points in 32-d.
+
+Workload (the 1M-point configuration the headline metric is quoted on that is
+GEMM-shaped): n = 1,000,000 points, d = 32, float32 i.i.d.
+N(0,1) from numpy's default_rng(3); exact single linkage through Boruvka's MST.
+A step is one full tree (scipy-format Z back in host memory) from points
+already resident in HBM (cuhc_linkage_device). The timed form is the mixed
+one: FP16 MFMA candidate tiles + certified FP32 recheck; the plain FP32 form is
+timed once beside it and the trees must be identical.
+
+Prints one JSON line (rank 0). --gpus > 1 is not supported (one GPU).
+
+    python bench.py [--steps K] [--warmup W] [--n N] [--plain-steps P]
+"""
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "s per full tree @1M pts + dist-tile MFMA util; mixed/learned vs plain FP32"
+FP16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 MFMA
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_points(n, d, seed):
+    return np.random.default_rng(seed).standard_normal((n, d), dtype=np.float32)
+
+
+def tree_on_device(lib, ptr, n, d, tree):
+    from cuhc_amd import _lib
+    _lib.check(lib.cuhc_linkage_device(ptr, n, d, 0, 2.0, 0, 1, _lib.double_ptr(tree)))
+
+
+def timed_steps(fn, steps):
+    # Each call is blocking: it returns after the tree is in host memory and
+    # the library's stream is idle, so wall time brackets exactly the steps.
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    return (time.perf_counter() - t0) / steps
+
+
+def cpu_baseline(d, seed, sample_n, full_n):
+    """The CPU oracle (Prim's MST in FP32, one thread) on a bounded prefix of the
+    same data; seconds per tree extrapolated to full_n by n^2 (its cost)."""
+    import oracle
+    X = make_points(full_n, d, seed)[:sample_n].copy()
+    t0 = time.perf_counter()
+    oracle.single_linkage(X)
+    t = time.perf_counter() - t0
+    return {"value": t * (full_n / sample_n) ** 2, "unit": "s", "cores": 1, "kind": "port",
+            "sample": f"oracle Prim single linkage on the first {sample_n} points "
+                      f"(d={d}) in {t:.2f} s, extrapolated to n={full_n} by n^2"}
+
+
+def main():
+    parser = argparse.ArgumentParser(description=__doc__)
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=3)
+    parser.add_argument("--warmup", type=int, default=1)
+    parser.add_argument("--n", type=int, default=1_000_000)
+    parser.add_argument("--d", type=int, default=32)
+    parser.add_argument("--seed", type=int, default=3)
+    parser.add_argument("--plain-steps", type=int, default=1)
+    parser.add_argument("--cpu-sample", type=int, default=20_000)
+    parser.add_argument("--no-cpu", action="store_true")
+    args = parser.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only (this build has no multi-GPU path); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import cuhc_amd  # noqa: F401
+    from cuhc_amd import _lib
+
+    n, d = args.n, args.d
+    log(f"generating n={n} d={d} seed={args.seed}")
+    X = make_points(n, d, args.seed)
+    dev = _lib.DeviceArray(X)  # inputs resident in HBM before the timed region
+    tree = np.empty((n - 1, 4), dtype=np.float64)
+    lib = _lib.lib
+
+    # Mixed form: warm-up, then K timed trees with per-phase HIP-event timing.
+    _lib.set_precision("mixed")
+    for i in range(args.warmup):
+        log(f"warmup {i + 1}/{args.warmup}")
+        tree_on_device(lib, dev.ptr, n, d, tree)
+    _lib.set_timing(True)
+    _lib.reset_timings()
+    _lib.mixed_stats()
+    log(f"timing {args.steps} steps (mixed)")
+    s_mixed = timed_steps(lambda: tree_on_device(lib, dev.ptr, n, d, tree), args.steps)
+    phases = _lib.read_timings()
+    stats = _lib.mixed_stats()
+    _lib.set_timing(False)
+    z_mixed = tree.copy()
+    log(f"mixed: {s_mixed:.3f} s/tree; phases {phases}; stats {stats}")
+
+    ms, launches, pairs = phases["mfma_candidates"]
+    kernel_ms = ms / max(launches, 1)
+    flops_per_launch = 2.0 * d * pairs / max(launches, 1)
+    achieved = flops_per_launch / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
+
+    result = {
+        "metric": METRIC, "value": s_mixed, "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": s_mixed * 1e3, "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None, "dtype": "fp16 mfma + fp32",
+        "data": "synthetic N(0,1) float32, seed 3",
+        "config": {"workload": f"single linkage (Boruvka MST), n={n}, d={d}, euclidean, "
+                               "points resident in HBM", "n": n, "d": d,
+                   "form": "mixed: FP16 MFMA candidate tiles + certified FP32 recheck",
+                   "parallelism": "1 GPU"},
+        "roofline": {"bound": "mfma", "kernel": "mfmaCandidateKernel<2>",
+                     "achieved": round(achieved, 2), "peak": FP16_MFMA_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP16_MFMA_PEAK_TFLOPS, 4),
+                     "traffic": None, "avg_kernel_ms": round(kernel_ms, 3),
+                     "launches_per_tree": launches / max(args.steps, 1),
+                     "algorithmic_flops_per_launch": flops_per_launch},
+        "phase_ms_per_tree": {k: round(v[0] / max(args.steps, 1), 3) for k, v in phases.items()
+                              if v[1]},
+        "mixed_stats_per_tree": {k: v / max(args.steps, 1) for k, v in stats.items()},
+    }
+
+    if args.plain_steps > 0:
+        _lib.set_precision("fp32")
+        log(f"timing {args.plain_steps} steps (plain fp32)")
+        s_plain = timed_steps(lambda: tree_on_device(lib, dev.ptr, n, d, tree),
+                              args.plain_steps)
+        result["plain_fp32_s_per_tree"] = s_plain
+        result["speedup_vs_plain"] = s_plain / s_mixed
+        result["tree_identical_to_plain"] = bool(np.array_equal(tree, z_mixed))
+        log(f"plain: {s_plain:.3f} s/tree, identical={result['tree_identical_to_plain']}")
+    _lib.set_precision("auto")
+
+    if not args.no_cpu:
+        log("cpu baseline")
+        result["cpu_baseline"] = cpu_baseline(d, args.seed, min(args.cpu_sample, n), n)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

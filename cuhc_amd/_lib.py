@@ -1,0 +1,150 @@
+"""ctypes binding of libcuhc.so (the gfx950 HIP library built by cuhc_amd.build).
+
+Mirrors the reference binding (reference python/cuhc/_lib.py): the library is
+loaded from this package's directory and importing fails loudly when it is not
+built. There is no CPU fallback.
+"""
+
+import ctypes
+import pathlib
+
+_LIB_PATH = pathlib.Path(__file__).resolve().parent / "libcuhc.so"
+
+if not _LIB_PATH.exists():
+    raise ImportError(
+        f"{_LIB_PATH} not found; build it with `python cuhc_amd/build.py` "
+        "(hipcc, gfx950) before importing cuhc_amd"
+    )
+
+lib = ctypes.CDLL(str(_LIB_PATH))
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u64p = ctypes.POINTER(ctypes.c_ulonglong)
+_int, _float, _i64 = ctypes.c_int, ctypes.c_float, ctypes.c_int64
+
+
+def _declare(name, restype, *argtypes):
+    fn = getattr(lib, name)
+    fn.restype = restype
+    fn.argtypes = list(argtypes)
+    return fn
+
+
+# The eight reference entry points (same signatures).
+_declare("cuhc_pairwise_distances", _int, _f32p, _int, _f32p, _int, _int, _int, _float, _f32p)
+_declare("cuhc_linkage", _int, _f32p, _int, _int, _int, _float, _int, _int, _f64p)
+_declare("cuhc_ward_rnn_with_options", _int, _f32p, _int, _int, _float, _int, _int, _f64p)
+_declare("cuhc_hdbscan_tree", _int, _f32p, _int, _int, _int, _float, _int, _float, _int, _f64p,
+         _f32p)
+_declare("cuhc_condense_tree", _i64, _f64p, _i64, _i64, _i64, _i64p, _i64p, _f64p, _i64p)
+_declare("cuhc_set_pruning", _int, _int)
+_declare("cuhc_prune_stats", None, _u64p, _u64p)
+_declare("cuhc_error_string", ctypes.c_char_p, _int)
+
+# Extensions of this build.
+_declare("cuhc_set_precision", _int, _int)
+_declare("cuhc_linkage_device", _int, ctypes.c_void_p, _int, _int, _int, _float, _int, _int,
+         _f64p)
+_declare("cuhc_hdbscan_tree_device", _int, ctypes.c_void_p, _int, _int, _int, _float, _int,
+         _float, _int, _f64p, _f32p)
+_declare("cuhc_device_alloc", _int, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_void_p))
+_declare("cuhc_device_free", _int, ctypes.c_void_p)
+_declare("cuhc_copy_to_device", _int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong)
+_declare("cuhc_copy_to_host", _int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong)
+_declare("cuhc_set_timing", _int, _int)
+_declare("cuhc_read_timings", None, _f64p, _i64p, _i64p, _int)
+_declare("cuhc_reset_timings", None)
+_declare("cuhc_mixed_stats", None, _u64p, _u64p, _u64p, _u64p)
+
+PRUNE_MODES = {"auto": 0, "always": 1, "never": 2}
+PRECISION_MODES = {"auto": 0, "fp32": 1, "mixed": 2}
+PHASES = ["distance_tiles", "mfma_candidates", "recheck", "core", "contract", "dense", "rnn",
+          "mlp"]
+
+# hipErrorOutOfMemory, the status the dense path returns when its matrix does not fit.
+HIP_ERROR_OUT_OF_MEMORY = 2
+
+
+def check(status):
+    """Raises RuntimeError for a non-zero status of a C ABI call."""
+    if status != 0:
+        raise RuntimeError(f"cuhc: {lib.cuhc_error_string(status).decode()} (code {status})")
+
+
+def set_pruning(mode):
+    """Spatial pruning for mst, rnn and HDBSCAN: "auto", "always" or "never" (same results)."""
+    check(lib.cuhc_set_pruning(PRUNE_MODES[mode]))
+
+
+def prune_stats():
+    """(tiles computed, tiles in total) of the pruned passes since the last call."""
+    computed, total = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+    lib.cuhc_prune_stats(ctypes.byref(computed), ctypes.byref(total))
+    return computed.value, total.value
+
+
+def set_precision(mode):
+    """Boruvka pass precision: "auto", "fp32" (plain tiles) or "mixed" (same trees)."""
+    check(lib.cuhc_set_precision(PRECISION_MODES[mode]))
+
+
+def mixed_stats():
+    """Counters of the mixed-precision passes since the last call (then reset)."""
+    vals = [ctypes.c_ulonglong() for _ in range(4)]
+    lib.cuhc_mixed_stats(*[ctypes.byref(v) for v in vals])
+    return dict(zip(["recorded", "rechecked_tiles", "rechecked_pairs", "overflow_rows"],
+                    [v.value for v in vals]))
+
+
+def set_timing(on):
+    lib.cuhc_set_timing(1 if on else 0)
+
+
+def reset_timings():
+    lib.cuhc_reset_timings()
+
+
+def read_timings():
+    """{phase: (total ms, launches, algorithmic units)} since the last reset."""
+    n = len(PHASES)
+    ms = (ctypes.c_double * n)()
+    launches = (ctypes.c_int64 * n)()
+    units = (ctypes.c_int64 * n)()
+    lib.cuhc_read_timings(ms, launches, units, n)
+    return {name: (ms[i], launches[i], units[i]) for i, name in enumerate(PHASES)}
+
+
+class DeviceArray:
+    """A host array copied into HBM through the library's own HIP runtime (no
+    torch: its bundled HIP runtime cannot share the process with this one)."""
+
+    def __init__(self, host):
+        import numpy as np
+        self.host = np.ascontiguousarray(host)
+        self.shape, self.dtype = self.host.shape, self.host.dtype
+        self.ptr = ctypes.c_void_p()
+        check(lib.cuhc_device_alloc(self.host.nbytes, ctypes.byref(self.ptr)))
+        check(lib.cuhc_copy_to_device(self.ptr, self.host.ctypes.data_as(ctypes.c_void_p),
+                                      self.host.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib.cuhc_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        self.free()
+
+
+def float_ptr(array):
+    return array.ctypes.data_as(_f32p)
+
+
+def double_ptr(array):
+    return array.ctypes.data_as(_f64p)
+
+
+def int64_ptr(array):
+    return array.ctypes.data_as(_i64p)

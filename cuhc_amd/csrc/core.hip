@@ -1,0 +1,300 @@
+// Core distances: core[i] = the k-th smallest distance from point i to the
+// points, the point itself counted as the 1st (distance 0), values clamped at
+// 0 (reference src/hdbscan.cuh:35-338).
+//
+// fused (k <= 64): one workgroup per row tile walks the column tiles nearest
+//   first and keeps each row's k smallest distances sorted in LDS. After each
+//   tile the 16 lanes of a row mark the distances that beat the row's k-th,
+//   and the row's lowest lane with work inserts one value per step (64-bit
+//   ballot over the wave: 4 rows insert in parallel). Optional box pruning for
+//   points in spatial order.
+// radix: distances of a block of rows into a bounded buffer, then per row a
+//   4-pass MSB-first byte radix select over the FP32 bit patterns with
+//   per-wave LDS histograms.
+// Both are exact and bit-identical (the k-th smallest value is unique).
+#include "mst.hpp"
+#include "tile.hpp"
+
+#include <algorithm>
+
+namespace cuhc {
+
+namespace {
+
+struct CoreBoxes {
+  const float *low = nullptr, *high = nullptr;
+  int numTiles = 0;
+};
+
+DEV int walkTileCore(int rowTile, int step, int numTiles) {
+  const int off = (step + 1) >> 1;
+  const int tile = (step & 1) ? rowTile + off : rowTile - off;
+  return (tile >= 0 && tile < numTiles) ? tile : -1;
+}
+
+HD float coreBoxSlack(int d) { return 1e-5f + (d + 8) * 5e-7f; }
+
+template <class Metric>
+DEV float coreBoxBound(const Metric &metric, const float *lowA, const float *highA,
+                       const float *lowB, const float *highB, int d) {
+  typename Metric::Acc acc = metric.init();
+  for (int k = 0; k < d; ++k) {
+    const float gap = fmaxf(0.0f, fmaxf(lowB[k] - highA[k], lowA[k] - highB[k]));
+    acc = metric.fold(acc, gap, 0.0f);
+  }
+  return metric.finish(acc);
+}
+
+// Inserts v into the ascending list[0..k), dropping the largest, if v beats list[k-1].
+DEV void insertSorted(float *list, int k, float v) {
+  if (!(v < list[k - 1]))
+    return;
+  int pos = k - 1;
+  while (pos > 0 && list[pos - 1] > v) {
+    list[pos] = list[pos - 1];
+    --pos;
+  }
+  list[pos] = v;
+}
+
+template <class T, class Metric>
+__global__ void __launch_bounds__(256)
+    kthNearestFusedKernel(const float *__restrict__ X, int n, int d, Metric metric, int k,
+                          float *__restrict__ core, CoreBoxes boxes) {
+  constexpr int TR = T::TR;
+  __shared__ __align__(16) TileSmem<T> sm;
+  __shared__ int skip;
+  extern __shared__ float lists[]; // [DIM][k], then the row tile's box (2d) if pruning
+
+  const int row0 = blockIdx.x * T::DIM;
+  for (int i = threadIdx.x; i < T::DIM * k; i += T::THREADS)
+    lists[i] = INFINITY;
+  const bool prune = boxes.numTiles > 0;
+  float *rowBox = lists + T::DIM * k;
+  if (prune)
+    for (int f = threadIdx.x; f < d; f += T::THREADS) {
+      rowBox[f] = boxes.low[(size_t)blockIdx.x * d + f];
+      rowBox[d + f] = boxes.high[(size_t)blockIdx.x * d + f];
+    }
+  __syncthreads();
+  const float slack = 1.0f - coreBoxSlack(d);
+  const int numTiles = (n + T::DIM - 1) / T::DIM;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long groupMask = 0xffffull << (lane & 48);
+
+  for (int step = 0; step < 2 * numTiles; ++step) {
+    const int colTile = walkTileCore(blockIdx.x, step, numTiles);
+    if (colTile < 0)
+      continue;
+    if (prune) {
+      __syncthreads(); // previous insertions done
+      if (threadIdx.x < 64) {
+        float largest = 0.0f;
+        for (int i = threadIdx.x; i < T::DIM; i += 64)
+          if (row0 + i < n)
+            largest = fmaxf(largest, lists[i * k + k - 1]);
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1)
+          largest = fmaxf(largest, __shfl_xor(largest, m, 64));
+        if (threadIdx.x == 0)
+          skip = largest != INFINITY &&
+                 coreBoxBound(metric, rowBox, rowBox + d, boxes.low + (size_t)colTile * d,
+                              boxes.high + (size_t)colTile * d, d) * slack > largest;
+      }
+      __syncthreads();
+      const bool doSkip = skip;
+      __syncthreads();
+      if (doSkip)
+        continue;
+    }
+    const int col0 = colTile * T::DIM;
+    float v[TR][TR];
+    computeTile<T>(sm, X, n, X, n, d, metric, row0, col0, v);
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+      const int rit = T::rowInTile(r);
+      const int row = row0 + rit;
+      float *list = lists + rit * k;
+      unsigned int pending = 0;
+      if (row < n) {
+        const float kth = list[k - 1];
+#pragma unroll
+        for (int c = 0; c < TR; ++c) {
+          const int col = col0 + T::colInTile(c);
+          // The point itself is distance 0 (drops float noise such as cosine's 1e-7).
+          v[r][c] = col == row ? 0.0f : fmaxf(v[r][c], 0.0f);
+          if (col < n && v[r][c] < kth)
+            pending |= 1u << c;
+        }
+      }
+      // One insertion per row per step until no lane of the wave has work;
+      // all lanes stay converged for the ballot.
+      while (true) {
+        const unsigned long long work = __ballot(pending != 0);
+        if (work == 0)
+          break;
+        const unsigned long long mine = work & groupMask;
+        if (mine != 0 && lane == __ffsll((long long)mine) - 1) {
+          const int c = __ffs(pending) - 1;
+          pending &= pending - 1;
+          float value = 0.0f;
+#pragma unroll
+          for (int cc = 0; cc < TR; ++cc)
+            if (cc == c)
+              value = v[r][cc];
+          insertSorted(list, k, value);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < T::DIM; i += T::THREADS)
+    if (row0 + i < n)
+      core[row0 + i] = lists[i * k + k - 1];
+}
+
+__global__ void coreBoxesKernel(const float *__restrict__ X, int n, int d, int tileDim,
+                                float *low, float *high) {
+  const int tile = blockIdx.x;
+  const int first = tile * tileDim, last = min(n, first + tileDim);
+  for (int f = threadIdx.x; f < d; f += blockDim.x) {
+    float lo = INFINITY, hi = -INFINITY;
+    for (int p = first; p < last; ++p) {
+      const float v = X[(size_t)p * d + f];
+      lo = fminf(lo, v);
+      hi = fmaxf(hi, v);
+    }
+    low[(size_t)tile * d + f] = lo;
+    high[(size_t)tile * d + f] = hi;
+  }
+}
+
+// One workgroup (256 threads) per row of the distance block.
+__global__ void __launch_bounds__(256)
+    kthSmallestRowKernel(const float *__restrict__ dist, int numCols, int firstRow, int k,
+                         float *__restrict__ core) {
+  __shared__ unsigned int hist[4][256];
+  __shared__ unsigned int prefix, prefixMask, remaining;
+  const int row = blockIdx.x, self = firstRow + row;
+  const float *rd = dist + (size_t)row * numCols;
+  const int wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) {
+    prefix = 0;
+    prefixMask = 0;
+    remaining = (unsigned)k;
+  }
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    for (int i = threadIdx.x; i < 4 * 256; i += 256)
+      (&hist[0][0])[i] = 0;
+    __syncthreads();
+    const unsigned int pre = prefix, mask = prefixMask;
+    for (int col = threadIdx.x; col < numCols; col += 256) {
+      const float value = col == self ? 0.0f : fmaxf(rd[col], 0.0f);
+      const unsigned int bits = __float_as_uint(value);
+      if ((bits & mask) == pre)
+        atomicAdd(&hist[wave][(bits >> shift) & 0xffu], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned int below = 0, bin = 0;
+      while (true) {
+        const unsigned int count = hist[0][bin] + hist[1][bin] + hist[2][bin] + hist[3][bin];
+        if (below + count >= remaining || bin == 255)
+          break;
+        below += count;
+        ++bin;
+      }
+      remaining -= below;
+      prefix |= bin << shift;
+      prefixMask |= 0xffu << shift;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    core[self] = __uint_as_float(prefix);
+}
+
+template <class F> int withMetricCore(MetricSpec m, F &&f) {
+  switch (m.id) {
+  case kEuclidean: return f(Euclidean{});
+  case kSqEuclidean: return f(SqEuclidean{});
+  case kManhattan: return f(Manhattan{});
+  case kChebyshev: return f(Chebyshev{});
+  case kMinkowski: return f(Minkowski{m.p});
+  case kCosine: return f(Cosine{});
+  default: return kErrInvalidMetric;
+  }
+}
+
+int coreFused(const float *X, int n, int d, MetricSpec m, int k, float *core, bool prune) {
+  return withMetricCore(m, [&](auto metric) {
+    using Metric = decltype(metric);
+    auto run = [&](auto fill) {
+      using T = typename TileFor<Metric, decltype(fill)::value>::type;
+      const int tiles = (n + T::DIM - 1) / T::DIM;
+      size_t smem = (size_t)T::DIM * k * sizeof(float);
+      CoreBoxes boxes;
+      DevArray<float> low, high;
+      Status s;
+      if (prune && Metric::kBoxBound) {
+        s(low.alloc((size_t)tiles * d));
+        s(high.alloc((size_t)tiles * d));
+        if (!s.ok())
+          return s.code;
+        hipLaunchKernelGGL(coreBoxesKernel, dim3(tiles), dim3(64), 0, stream(), X, n, d, T::DIM,
+                           low.get(), high.get());
+        boxes = CoreBoxes{low, high, tiles};
+        smem += 2 * d * sizeof(float);
+      }
+      PhaseTimer timer(kPhaseCore);
+      hipLaunchKernelGGL((kthNearestFusedKernel<T, Metric>), dim3(tiles), dim3(256), smem,
+                         stream(), X, n, d, metric, k, core, boxes);
+      s((int)hipGetLastError());
+      s(sync());
+      return s.code;
+    };
+    if (d <= 8)
+      return run(std::integral_constant<int, 8>{});
+    return run(std::integral_constant<int, 32>{});
+  });
+}
+
+int coreRadix(const float *X, int n, int d, MetricSpec m, int k, float *core) {
+  const size_t maxBuffer = (size_t)512 << 20;
+  const int tileRows = tilePoints(m.id);
+  const size_t rowBytes = (size_t)n * sizeof(float);
+  int rows = (int)std::min<size_t>(maxBuffer / rowBytes, (size_t)n);
+  rows = std::max(tileRows, rows / tileRows * tileRows);
+  rows = std::min(rows, n);
+  DevArray<float> chunk;
+  Status s;
+  s(chunk.alloc((size_t)rows * n));
+  PhaseTimer timer(kPhaseCore);
+  for (int first = 0; s.ok() && first < n; first += rows) {
+    const int count = std::min(rows, n - first);
+    s(pairwiseMatrix(X + (size_t)first * d, count, X, n, d, m, chunk));
+    hipLaunchKernelGGL(kthSmallestRowKernel, dim3(count), dim3(256), 0, stream(), chunk.get(), n,
+                       first, k, core);
+    s((int)hipGetLastError());
+  }
+  s(sync());
+  return s.code;
+}
+
+} // namespace
+
+int coreDistances(const float *X, int n, int d, MetricSpec m, int k, float *core, int method,
+                  bool prune) {
+  if (k < 1 || k > n)
+    return kErrInvalidShape;
+  if (method == kCoreFused && k > kMaxFusedK)
+    return kErrInvalidAlgorithm;
+  if (method == kCoreFused || (method == kCoreAuto && k <= kMaxFusedK))
+    return coreFused(X, n, d, m, k, core, prune);
+  return coreRadix(X, n, d, m, k, core);
+}
+
+} // namespace cuhc

@@ -1,0 +1,648 @@
+// All-pairs FP32 kernels: distance matrices (full and condensed), the
+// symmetric nearest-other-component pass, and the spatially pruned walk pass.
+//
+// Reference counterparts: pairwiseDistanceKernel and
+// nearestOtherComponentSymmetricKernel (src/distance_matrix.cuh:179-200,
+// 287-390), buildTileBoxesKernel / nearestOtherComponentWalkKernel /
+// estimatePrunedShare (src/spatial.cuh:136-343, 424-469). The tile fold is
+// tile.hpp; argmins carry (weight, index) as one u64 key.
+#include "passes.hpp"
+#include "tile.hpp"
+
+#include <algorithm>
+#include <numeric>
+
+namespace cuhc {
+
+bool validMetric(int id) { return id >= kEuclidean && id <= kCosine; }
+bool metricHasBoxBound(int id) { return validMetric(id) && id != kCosine; }
+int tilePoints(int metricId) { return metricId == kCosine ? 64 : 128; }
+
+namespace {
+
+template <class F> int withMetric(MetricSpec m, F &&f) {
+  switch (m.id) {
+  case kEuclidean: return f(Euclidean{});
+  case kSqEuclidean: return f(SqEuclidean{});
+  case kManhattan: return f(Manhattan{});
+  case kChebyshev: return f(Chebyshev{});
+  case kMinkowski: return f(Minkowski{m.p});
+  case kCosine: return f(Cosine{});
+  default: return kErrInvalidMetric;
+  }
+}
+
+// LDS staging width: 8 features for d <= 8, else 32 (a fill stages only the
+// features present, rounded up to 8).
+template <class F> int withFill(int d, F &&f) {
+  if (d <= 8)
+    return f(std::integral_constant<int, 8>{});
+  return f(std::integral_constant<int, 32>{});
+}
+
+int persistentBlocks(long long work, int perCU) {
+  const long long cap = (long long)numCUs() * perCU;
+  return (int)std::max<long long>(1, std::min<long long>(work, cap));
+}
+
+// ---------------------------------------------------------------------------
+// Distance matrices
+
+template <class T, class Metric>
+__global__ void __launch_bounds__(256)
+    pairwiseMatrixKernel(const float *__restrict__ A, int nA, const float *__restrict__ B, int nB,
+                         int d, Metric metric, float *__restrict__ out, int tilesB) {
+  __shared__ __align__(16) TileSmem<T> sm;
+  const int tileI = blockIdx.x / tilesB, tileJ = blockIdx.x - tileI * tilesB;
+  const int row0 = tileI * T::DIM, col0 = tileJ * T::DIM;
+  float v[T::TR][T::TR];
+  computeTile<T>(sm, A, nA, B, nB, d, metric, row0, col0, v);
+#pragma unroll
+  for (int r = 0; r < T::TR; ++r) {
+    const int row = row0 + T::rowInTile(r);
+    if (row >= nA)
+      continue;
+#pragma unroll
+    for (int c = 0; c < T::TR; ++c) {
+      const int col = col0 + T::colInTile(c);
+      if (col < nB)
+        out[(size_t)row * nB + col] = v[r][c];
+    }
+  }
+}
+
+template <class T, class Metric>
+__global__ void __launch_bounds__(256)
+    condensedMatrixKernel(const float *__restrict__ X, int n, int d, Metric metric,
+                          float *__restrict__ out, long long numPairs) {
+  __shared__ __align__(16) TileSmem<T> sm;
+  for (long long t = blockIdx.x; t < numPairs; t += gridDim.x) {
+    int I, J;
+    decodeTrianglePair(t, I, J);
+    const int row0 = I * T::DIM, col0 = J * T::DIM;
+    float v[T::TR][T::TR];
+    computeTile<T>(sm, X, n, X, n, d, metric, row0, col0, v);
+#pragma unroll
+    for (int r = 0; r < T::TR; ++r) {
+      const long long i = row0 + T::rowInTile(r);
+      if (i >= n)
+        continue;
+      const long long base = i * n - i * (i + 1) / 2 - i - 1;
+#pragma unroll
+      for (int c = 0; c < T::TR; ++c) {
+        const long long j = col0 + T::colInTile(c);
+        if (j > i && j < n)
+          out[base + j] = v[r][c];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Symmetric nearest-other-component pass
+
+// Reduces a key over the 16 lanes of a tile row (xor 8, 4, 2, 1).
+DEV unsigned long long rowGroupMin(unsigned long long k) {
+#pragma unroll
+  for (int m = 8; m > 0; m >>= 1)
+    k = minKey(k, shflXorKey(k, m));
+  return k;
+}
+
+template <class T, class Metric, class Weight>
+__global__ void __launch_bounds__(256)
+    nearestOtherSymmetricKernel(const float *__restrict__ X, int n, int d, Metric metric,
+                                Weight weight, const int *__restrict__ comp,
+                                const int *__restrict__ orig,
+                                unsigned long long *__restrict__ nearest, long long numPairs) {
+  constexpr int TR = T::TR;
+  __shared__ __align__(16) TileSmem<T> sm;
+  __shared__ unsigned long long colBest[T::DIM];
+
+  for (long long t = blockIdx.x; t < numPairs; t += gridDim.x) {
+    int I, J;
+    decodeTrianglePair(t, I, J);
+    const int row0 = I * T::DIM, col0 = J * T::DIM;
+    for (int i = threadIdx.x; i < T::DIM; i += T::THREADS)
+      colBest[i] = kEmptyKey; // visible after computeTile's barriers
+
+    float w[TR][TR];
+    computeTile<T>(sm, X, n, X, n, d, metric, row0, col0, w);
+
+    int colComp[TR];
+    unsigned int colId[TR];
+    typename Weight::Value colVal[TR];
+#pragma unroll
+    for (int c = 0; c < TR; ++c) {
+      const int col = col0 + T::colInTile(c);
+      colComp[c] = col < n ? comp[col] : -1;
+      colVal[c] = weight.value(col < n ? col : 0);
+      colId[c] = (unsigned)(orig && col < n ? orig[col] : col);
+    }
+    // A pair counts if both points exist, lie in different components and
+    // have a finite weight (+inf never counts, as in the reference's
+    // symmetric pass; NaN counts and packs as 0).
+    unsigned long long rowKey[TR];
+    unsigned long long countMask = 0;
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+      const int row = row0 + T::rowInTile(r);
+      const int rc = row < n ? comp[row] : -2;
+      const typename Weight::Value rv = weight.value(row < n ? row : 0);
+      unsigned long long best = kEmptyKey;
+#pragma unroll
+      for (int c = 0; c < TR; ++c) {
+        const float ww = weight(w[r][c], rv, colVal[c]);
+        const bool counts = colComp[c] >= 0 && rc >= 0 && colComp[c] != rc && ww != INFINITY;
+        w[r][c] = ww;
+        if (counts) {
+          countMask |= 1ull << (r * TR + c);
+          best = minKey(best, packKey(ww, colId[c]));
+        }
+      }
+      rowKey[r] = rowGroupMin(best);
+    }
+    if (T::tc() == 0) {
+#pragma unroll
+      for (int r = 0; r < TR; ++r) {
+        const int row = row0 + T::rowInTile(r);
+        if (row < n && rowKey[r] != kEmptyKey)
+          atomicMin(&nearest[row], rowKey[r]);
+      }
+    }
+    if (I != J) {
+      // Columns: this thread's rows, then lanes tc+16, +32, +48 of the wave,
+      // then the 4 waves through LDS.
+      unsigned int rowId[TR];
+#pragma unroll
+      for (int r = 0; r < TR; ++r) {
+        const int row = row0 + T::rowInTile(r);
+        rowId[r] = (unsigned)(orig && row < n ? orig[row] : row);
+      }
+#pragma unroll
+      for (int c = 0; c < TR; ++c) {
+        unsigned long long best = kEmptyKey;
+#pragma unroll
+        for (int r = 0; r < TR; ++r)
+          if (countMask & (1ull << (r * TR + c)))
+            best = minKey(best, packKey(w[r][c], rowId[r]));
+        best = minKey(best, shflXorKey(best, 16));
+        best = minKey(best, shflXorKey(best, 32));
+        if ((threadIdx.x & 63) < 16 && best != kEmptyKey)
+          atomicMin(&colBest[T::colInTile(c)], best);
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < T::DIM; i += T::THREADS) {
+        const int col = col0 + i;
+        if (col < n && colBest[i] != kEmptyKey)
+          atomicMin(&nearest[col], colBest[i]);
+      }
+    }
+    __syncthreads(); // colBest reused by the next tile
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Spatial pruning: boxes and the walk pass
+
+struct Boxes {
+  float *low = nullptr, *high = nullptr; // tiles x d
+  float *minValue = nullptr;             // smallest per-point weight value in the tile
+  int *minComp = nullptr, *maxComp = nullptr;
+  int numTiles = 0;
+};
+
+template <class Weight>
+__global__ void buildBoxesKernel(const float *__restrict__ X, int n, int d, int tileDim,
+                                 Weight weight, const int *__restrict__ comp, Boxes boxes) {
+  const int tile = blockIdx.x;
+  const int first = tile * tileDim, last = min(n, first + tileDim);
+  for (int f = threadIdx.x; f < d; f += blockDim.x) {
+    float lo = INFINITY, hi = -INFINITY;
+    for (int p = first; p < last; ++p) {
+      const float v = X[(size_t)p * d + f];
+      lo = fminf(lo, v);
+      hi = fmaxf(hi, v);
+    }
+    boxes.low[(size_t)tile * d + f] = lo;
+    boxes.high[(size_t)tile * d + f] = hi;
+  }
+  // Per-point values and component range: a wave-wide min/max.
+  if (threadIdx.x < 64) {
+    float mv = INFINITY;
+    int cmin = 0x7fffffff, cmax = -1;
+    for (int p = first + (int)threadIdx.x; p < last; p += 64) {
+      mv = fminf(mv, (float)weight.value(p));
+      const int c = comp ? comp[p] : 0;
+      cmin = min(cmin, c);
+      cmax = max(cmax, c);
+    }
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+      mv = fminf(mv, __shfl_xor(mv, m, 64));
+      cmin = min(cmin, __shfl_xor(cmin, m, 64));
+      cmax = max(cmax, __shfl_xor(cmax, m, 64));
+    }
+    if (threadIdx.x == 0) {
+      boxes.minValue[tile] = Weight::kHasValue ? mv : 0.0f;
+      boxes.minComp[tile] = cmin;
+      boxes.maxComp[tile] = cmax;
+    }
+  }
+}
+
+// Relative slack of the box bound (reference src/spatial.cuh:183-185): far
+// larger than the rounding of a pair's distance or of the bound, far too
+// small to cost pruning.
+HD float boxSlack(int d) { return 1e-5f + (d + 8) * 5e-7f; }
+
+template <class Metric>
+DEV float boxLowerBound(const Metric &metric, const float *lowA, const float *highA,
+                        const float *lowB, const float *highB, int d) {
+  typename Metric::Acc acc = metric.init();
+  for (int k = 0; k < d; ++k) {
+    const float gap = fmaxf(0.0f, fmaxf(lowB[k] - highA[k], lowA[k] - highB[k]));
+    acc = metric.fold(acc, gap, 0.0f);
+  }
+  return metric.finish(acc);
+}
+
+// Column tile visited at `step` from row tile I: I, I+1, I-1, I+2, ... (-1 if outside).
+DEV int walkTile(int rowTile, int step, int numTiles) {
+  const int off = (step + 1) >> 1;
+  const int tile = (step & 1) ? rowTile + off : rowTile - off;
+  return (tile >= 0 && tile < numTiles) ? tile : -1;
+}
+
+template <class T, class Metric, class Weight>
+__global__ void __launch_bounds__(256)
+    nearestOtherWalkKernel(const float *__restrict__ X, int n, int d, Metric metric,
+                           Weight weight, const int *__restrict__ comp,
+                           const int *__restrict__ orig, Boxes boxes,
+                           unsigned long long *__restrict__ nearest,
+                           unsigned long long *tilesComputed, int rowTileStride) {
+  constexpr int TR = T::TR;
+  __shared__ __align__(16) TileSmem<T> sm;
+  __shared__ unsigned long long rowBest[T::DIM];
+  __shared__ float bound;
+  __shared__ int skip;
+  extern __shared__ float rowBox[]; // low[d], high[d]
+
+  const int rowTile = blockIdx.x * rowTileStride, numTiles = boxes.numTiles;
+  const int row0 = rowTile * T::DIM;
+  for (int i = threadIdx.x; i < T::DIM; i += T::THREADS)
+    rowBest[i] = kEmptyKey;
+  for (int k = threadIdx.x; k < d; k += T::THREADS) {
+    rowBox[k] = boxes.low[(size_t)rowTile * d + k];
+    rowBox[d + k] = boxes.high[(size_t)rowTile * d + k];
+  }
+  if (threadIdx.x == 0)
+    bound = INFINITY;
+  const bool rowTilePure = boxes.minComp[rowTile] == boxes.maxComp[rowTile];
+  const float slack = 1.0f - boxSlack(d);
+
+  int rowComp[TR];
+  typename Weight::Value rowVal[TR];
+#pragma unroll
+  for (int r = 0; r < TR; ++r) {
+    const int row = row0 + T::rowInTile(r);
+    rowComp[r] = row < n ? comp[row] : -2;
+    rowVal[r] = weight.value(row < n ? row : 0);
+  }
+  unsigned long long computed = 0;
+  __syncthreads();
+
+  for (int step = 0; step < 2 * numTiles; ++step) {
+    const int colTile = walkTile(rowTile, step, numTiles);
+    if (colTile < 0)
+      continue; // uniform across the workgroup
+    if (threadIdx.x == 0) {
+      bool canSkip = rowTilePure && boxes.minComp[colTile] == boxes.maxComp[colTile] &&
+                     boxes.minComp[colTile] == boxes.minComp[rowTile];
+      if (!canSkip && bound != INFINITY) {
+        const float db = boxLowerBound(metric, rowBox, rowBox + d, boxes.low + (size_t)colTile * d,
+                                       boxes.high + (size_t)colTile * d, d);
+        const float wb = weight(db, boxes.minValue[rowTile], boxes.minValue[colTile]) * slack;
+        canSkip = wb > bound;
+      }
+      skip = canSkip;
+    }
+    __syncthreads();
+    const bool doSkip = skip;
+    __syncthreads(); // everyone has read skip
+    if (doSkip)
+      continue;
+    ++computed;
+
+    const int col0 = colTile * T::DIM;
+    float w[TR][TR];
+    computeTile<T>(sm, X, n, X, n, d, metric, row0, col0, w);
+    int colComp[TR];
+    typename Weight::Value colVal[TR];
+    unsigned int colIdx[TR];
+#pragma unroll
+    for (int c = 0; c < TR; ++c) {
+      const int col = col0 + T::colInTile(c);
+      colComp[c] = col < n ? comp[col] : -1;
+      colVal[c] = weight.value(col < n ? col : 0);
+      colIdx[c] = col < n ? (unsigned)orig[col] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+      unsigned long long best = kEmptyKey;
+#pragma unroll
+      for (int c = 0; c < TR; ++c) {
+        const float ww = weight(w[r][c], rowVal[r], colVal[c]);
+        const bool counts = rowComp[r] >= 0 && colComp[c] >= 0 && colComp[c] != rowComp[r] &&
+                            ww != INFINITY;
+        if (counts)
+          best = minKey(best, packKey(ww, colIdx[c]));
+      }
+      best = rowGroupMin(best);
+      if (T::tc() == 0 && best != kEmptyKey) {
+        const int i = T::rowInTile(r);
+        rowBest[i] = minKey(rowBest[i], best);
+      }
+    }
+    __syncthreads();
+    // New bound: the largest current best over the valid rows (inf while a
+    // row has none yet). Wave 0 reduces the DIM values.
+    if (threadIdx.x < 64) {
+      float largest = 0.0f;
+      for (int i = threadIdx.x; i < T::DIM; i += 64)
+        if (row0 + i < n)
+          largest = fmaxf(largest, rowBest[i] == kEmptyKey ? INFINITY : keyWeight(rowBest[i]));
+#pragma unroll
+      for (int m = 32; m > 0; m >>= 1)
+        largest = fmaxf(largest, __shfl_xor(largest, m, 64));
+      if (threadIdx.x == 0)
+        bound = largest;
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < T::DIM; i += T::THREADS)
+    if (row0 + i < n && rowBest[i] != kEmptyKey)
+      atomicMin(&nearest[row0 + i], rowBest[i]);
+  if (tilesComputed && threadIdx.x == 0)
+    atomicAdd(tilesComputed, computed);
+}
+
+__global__ void gatherRowsKernel(const float *__restrict__ src, const int *__restrict__ order,
+                                 int n, int d, float *__restrict__ dst) {
+  const size_t total = (size_t)n * d;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const size_t row = e / d, f = e - row * d;
+    dst[e] = src[(size_t)order[row] * d + f];
+  }
+}
+
+__global__ void iotaKernel(int *out, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = i;
+}
+
+template <class Metric, class F> int withWeightFor(const Metric &, WeightSpec w, F &&f) {
+  switch (w.kind) {
+  case kWeightPlain: return f(PlainWeight{});
+  case kWeightMReach: return f(MReachWeight{w.core, w.alpha});
+  default: return kErrInvalidAlgorithm;
+  }
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Host entry points
+
+int pairwiseMatrix(const float *A, int nA, const float *B, int nB, int d, MetricSpec m,
+                   float *out) {
+  return withMetric(m, [&](auto metric) {
+    using Metric = decltype(metric);
+    return withFill(d, [&](auto fill) {
+      using T = typename TileFor<Metric, decltype(fill)::value>::type;
+      const int tilesA = (nA + T::DIM - 1) / T::DIM, tilesB = (nB + T::DIM - 1) / T::DIM;
+      const long long blocks = (long long)tilesA * tilesB;
+      if (blocks > 0x7fffffffLL)
+        return (int)hipErrorInvalidConfiguration;
+      hipLaunchKernelGGL((pairwiseMatrixKernel<T, Metric>), dim3((unsigned)blocks), dim3(256), 0,
+                         stream(), A, nA, B, nB, d, metric, out, tilesB);
+      return (int)hipGetLastError();
+    });
+  });
+}
+
+int condensedMatrix(const float *X, int n, int d, MetricSpec m, float *out) {
+  return withMetric(m, [&](auto metric) {
+    using Metric = decltype(metric);
+    return withFill(d, [&](auto fill) {
+      using T = typename TileFor<Metric, decltype(fill)::value>::type;
+      const long long tiles = (n + T::DIM - 1) / T::DIM;
+      const long long pairs = tiles * (tiles + 1) / 2;
+      hipLaunchKernelGGL((condensedMatrixKernel<T, Metric>), dim3(persistentBlocks(pairs, 8)),
+                         dim3(256), 0, stream(), X, n, d, metric, out, pairs);
+      return (int)hipGetLastError();
+    });
+  });
+}
+
+int nearestOtherSymmetric(const float *X, int n, int d, MetricSpec m, WeightSpec w,
+                          const int *comp, const int *orig, unsigned long long *nearest) {
+  if (w.kind == kWeightWard) {
+    if (m.id != kSqEuclidean)
+      return kErrInvalidMetric;
+    WardWeight ward{w.size};
+    return withFill(d, [&](auto fill) {
+      using T = Tile<8, decltype(fill)::value>;
+      const long long tiles = (n + T::DIM - 1) / T::DIM, pairs = tiles * (tiles + 1) / 2;
+      hipLaunchKernelGGL((nearestOtherSymmetricKernel<T, SqEuclidean, WardWeight>),
+                         dim3(persistentBlocks(pairs, 8)), dim3(256), 0, stream(), X, n, d,
+                         SqEuclidean{}, ward, comp, orig, nearest, pairs);
+      return (int)hipGetLastError();
+    });
+  }
+  return withMetric(m, [&](auto metric) {
+    using Metric = decltype(metric);
+    return withWeightFor(metric, w, [&](auto weight) {
+      using Weight = decltype(weight);
+      return withFill(d, [&](auto fill) {
+        using T = typename TileFor<Metric, decltype(fill)::value>::type;
+        const long long tiles = (n + T::DIM - 1) / T::DIM, pairs = tiles * (tiles + 1) / 2;
+        PhaseTimer timer(kPhaseDistanceTiles);
+        addUnits(kPhaseDistanceTiles, (long long)n * (n + 1) / 2);
+        hipLaunchKernelGGL((nearestOtherSymmetricKernel<T, Metric, Weight>),
+                           dim3(persistentBlocks(pairs, 8)), dim3(256), 0, stream(), X, n, d,
+                           metric, weight, comp, orig, nearest, pairs);
+        return (int)hipGetLastError();
+      });
+    });
+  });
+}
+
+bool shouldPrune(int mode, int metricId, int n, int d) {
+  const int tiles = (n + 127) / 128;
+  if (!metricHasBoxBound(metricId) || mode == kPruneNever || tiles < 2 ||
+      d > kPruneHardMaxFeatures)
+    return false;
+  return mode == kPruneAlways || (n >= kPruneMinPoints && d <= kPruneMaxFeatures);
+}
+
+// k-d order: split each range at a tile boundary near its middle along the
+// feature with the widest spread (estimated from a sample), until each range
+// is one tile (the behaviour of reference src/spatial.cuh:48-91). The order
+// inside the partitions of nth_element is implementation-defined; ties between
+// equal weights always break by original index, so results do not depend on it.
+std::vector<int> spatialOrder(const float *pts, int n, int d) {
+  constexpr int kTile = 128, kSample = 64;
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  struct Range {
+    int begin, end;
+  };
+  std::vector<Range> stack{{0, n}};
+  while (!stack.empty()) {
+    const Range range = stack.back();
+    stack.pop_back();
+    const int length = range.end - range.begin;
+    if (length <= kTile)
+      continue;
+    int split = 0;
+    float widest = -1.0f;
+    const int step = std::max(1, length / kSample);
+    for (int f = 0; f < d; ++f) {
+      float lo = INFINITY, hi = -INFINITY;
+      for (int i = range.begin; i < range.end; i += step) {
+        const float v = pts[(size_t)order[i] * d + f];
+        lo = std::min(lo, v);
+        hi = std::max(hi, v);
+      }
+      if (hi - lo > widest) {
+        widest = hi - lo;
+        split = f;
+      }
+    }
+    const int tiles = (length + kTile - 1) / kTile;
+    const int middle = range.begin + tiles / 2 * kTile;
+    std::nth_element(order.begin() + range.begin, order.begin() + middle,
+                     order.begin() + range.end, [&](int a, int b) {
+                       return pts[(size_t)a * d + split] < pts[(size_t)b * d + split];
+                     });
+    stack.push_back({range.begin, middle});
+    stack.push_back({middle, range.end});
+  }
+  return order;
+}
+
+int gatherRows(const float *src, const int *order, int n, int d, float *dst) {
+  hipLaunchKernelGGL(gatherRowsKernel, dim3(std::max(1, std::min(4096, (n * d + 255) / 256))),
+                     dim3(256), 0, stream(), src, order, n, d, dst);
+  return (int)hipGetLastError();
+}
+
+namespace {
+
+struct DeviceBoxes {
+  DevArray<float> low, high, minValue;
+  DevArray<int> minComp, maxComp;
+  Boxes view;
+  int alloc(int tiles, int d) {
+    Status s;
+    s(low.alloc((size_t)tiles * d));
+    s(high.alloc((size_t)tiles * d));
+    s(minValue.alloc(tiles));
+    s(minComp.alloc(tiles));
+    s(maxComp.alloc(tiles));
+    view = Boxes{low, high, minValue, minComp, maxComp, tiles};
+    return s.code;
+  }
+};
+
+template <class T, class Metric, class Weight>
+int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &weight,
+            const int *comp, const int *orig, unsigned long long *nearest, double *share,
+            PassStats *stats, int stride) {
+  const int tiles = (n + T::DIM - 1) / T::DIM;
+  DeviceBoxes boxes;
+  DevArray<unsigned long long> counter;
+  Status s;
+  s(boxes.alloc(tiles, d));
+  s(counter.alloc(1));
+  if (!s.ok())
+    return s.code;
+  s(fill(counter, 0, sizeof(unsigned long long)));
+  hipLaunchKernelGGL((buildBoxesKernel<Weight>), dim3(tiles), dim3(64), 0, stream(), X, n, d,
+                     T::DIM, weight, comp, boxes.view);
+  s((int)hipGetLastError());
+  const int sampled = (tiles + stride - 1) / stride;
+  {
+    PhaseTimer timer(kPhaseDistanceTiles);
+    hipLaunchKernelGGL((nearestOtherWalkKernel<T, Metric, Weight>), dim3(sampled), dim3(256),
+                       2 * d * sizeof(float), stream(), X, n, d, metric, weight, comp, orig,
+                       boxes.view, nearest, counter.get(), stride);
+    s((int)hipGetLastError());
+  }
+  unsigned long long computed = 0;
+  s(download(&computed, counter, sizeof(computed)));
+  const unsigned long long total = (unsigned long long)sampled * tiles;
+  *share = total ? (double)computed / (double)total : 1.0;
+  addUnits(kPhaseDistanceTiles, (long long)computed * T::DIM * T::DIM);
+  if (stats && s.ok()) {
+    stats->tilesComputed += computed;
+    stats->tilesTotal += total;
+  }
+  return s.code;
+}
+
+} // namespace
+
+int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, const int *comp,
+                     const int *orig, unsigned long long *nearest, double *share,
+                     PassStats *stats, int stride) {
+  if (!metricHasBoxBound(m.id))
+    return kErrInvalidMetric;
+  if (w.kind == kWeightWard) {
+    if (m.id != kSqEuclidean)
+      return kErrInvalidMetric;
+    return withFill(d, [&](auto fill) {
+      using T = Tile<8, decltype(fill)::value>;
+      return runWalk<T>(X, n, d, SqEuclidean{}, WardWeight{w.size}, comp, orig, nearest, share,
+                        stats, stride);
+    });
+  }
+  return withMetric(m, [&](auto metric) {
+    using Metric = decltype(metric);
+    if constexpr (!Metric::kBoxBound) {
+      return kErrInvalidMetric;
+    } else {
+      return withWeightFor(metric, w, [&](auto weight) {
+        return withFill(d, [&](auto fill) {
+          using T = Tile<8, decltype(fill)::value>;
+          return runWalk<T>(X, n, d, metric, weight, comp, orig, nearest, share, stats, stride);
+        });
+      });
+    }
+  });
+}
+
+int estimatePrunedShare(const float *X, int n, int d, MetricSpec m, double *share) {
+  constexpr int kSampleTiles = 32;
+  const int tiles = (n + 127) / 128;
+  const int stride = std::max(1, tiles / kSampleTiles);
+  DevArray<int> comp, orig;
+  DevArray<unsigned long long> nearest;
+  Status s;
+  s(comp.alloc(n));
+  s(orig.alloc(n));
+  s(nearest.alloc(n));
+  if (!s.ok())
+    return s.code;
+  hipLaunchKernelGGL(iotaKernel, dim3(std::min(1024, (n + 255) / 256)), dim3(256), 0, stream(),
+                     comp.get(), n);
+  hipLaunchKernelGGL(iotaKernel, dim3(std::min(1024, (n + 255) / 256)), dim3(256), 0, stream(),
+                     orig.get(), n);
+  s(fill(nearest, 0xff, nearest.bytes()));
+  s(nearestOtherWalk(X, n, d, m, WeightSpec{}, comp, orig, nearest, share, nullptr, stride));
+  return s.code;
+}
+
+} // namespace cuhc

@@ -1,0 +1,640 @@
+// Mixed-precision nearest-other-component pass for Boruvka (euclidean and
+// squared euclidean): FP16 Gram candidates on MFMA with a certified per-row
+// error bound, then an exact FP32 recheck of every candidate. No reference
+// counterpart; the result equals the FP32 direct-form pass bit for bit.
+//
+// Scaled points x' = (x - mu) * s (mu the mean, s a power of two with
+// max |x'| <= 2^14) are rounded to FP16: xh. Per point: N = |xh|^2 (FP32),
+// R >= |xh|, rho >= |xh - x'| (exact FP16 rounding error, FP64).
+//
+// MFMA (v_mfma_f32_32x32x16_f16): for a block of 32 columns j and 32 rows i,
+//   v_ij = N_j + sum_k xh_jk * (-2 xh_ik)      (C seeded with N_j)
+// so S_ij = v_ij + N_i estimates |xh_i - xh_j|^2 with
+//   |S_ij - |xh_i - xh_j|^2| <= E_i = (K + 8) 2^-23 (R_i + R_max)^2
+// (every FP32 addition may round; K products are exact), and the exact scaled
+// distance r = |x'_i - x'_j| = s |x_i - x_j| satisfies | r - |xh_i - xh_j| |
+// <= delta_i = rho_i + rho_max. The FP32 direct-form value sigma of the pair
+// (the fma fold) is within (d + 4) 2^-24 of the exact squared distance.
+// Per row, from the running minimum m of v over valid columns:
+//   A     = sqrt(m + N_i + E_i) + delta_i            >= r of the best pair
+//   theta = (A sqrt(kappa) + delta_i)^2 + E_i - N_i
+// with kappa covering both FP32 roundings and a 2^-21 margin for ties of
+// sqrtf. A column with v_ij > theta has a larger FP32 value than the row's
+// best (or than its sqrt ties), so it cannot be the row's argmin; every
+// column with v_ij <= theta is a candidate.
+//
+// Rows are the points still to be searched this round (in component order);
+// columns are all points permuted so that every component is a contiguous
+// range, which makes the same-component mask a per-row interval: column tiles
+// inside every row's own component are skipped, tiles crossing one are
+// masked per value, all others need no mask. Each wave owns 128 rows (4 row
+// blocks of 32, the MFMA N dimension); the workgroup's 4 waves share column
+// tiles of 128 points staged in LDS (double-buffered). Per 128-column tile and
+// row the kernel records (tile, tile minimum) when the tile minimum is within
+// theta; the recheck kernel recomputes the FP32 direct-form weights of the
+// recorded tiles still within the final theta and keeps the exact argmin
+// (weight, original index). Rows whose list overflows are rechecked in full.
+#include "mst.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <numeric>
+
+namespace cuhc {
+
+namespace {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kTileCols = 128;
+constexpr int kWaveRows = 128;     // 4 row blocks of 32
+constexpr int kBlockRows = 4 * kWaveRows;
+constexpr int kCandCap = 32;
+
+std::atomic<int> gLayoutChecked{0}; // 0 unknown, 1 ok, -1 mismatch
+unsigned long long gStats[4];
+
+// D register r of lane l holds row m = (r / 4) * 8 + (l / 32) * 4 + r % 4 and
+// column n = l % 32; lane l supplies A[l % 32][8 (l / 32) .. +7] and
+// B[8 (l / 32) .. +7][l % 32]. Checked once per process by layoutProbe.
+DEV int mOfReg(int r, int lane) { return (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); }
+
+__global__ void layoutProbeKernel(int *bad) {
+  const int lane = threadIdx.x;
+  half8 a, b;
+  for (int i = 0; i < 8; ++i) {
+    const int k = 8 * (lane >> 5) + i;
+    a[i] = (_Float16)(k == 0 ? (float)(lane & 31) : (k == 1 ? 1.0f : 0.0f));
+    b[i] = (_Float16)(k == 0 ? 1.0f : (k == 1 ? 32.0f * (lane & 31) : 0.0f));
+  }
+  f32x16 c;
+  for (int r = 0; r < 16; ++r)
+    c[r] = 0.0f;
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+  int wrong = 0;
+  for (int r = 0; r < 16; ++r)
+    wrong += c[r] != (float)(mOfReg(r, lane) + 32 * (lane & 31));
+  if (wrong)
+    atomicAdd(bad, wrong);
+}
+
+// ---------------------------------------------------------------------------
+// Preparation: mean, scale, FP16 points and per-point bounds.
+
+__global__ void columnSumsKernel(const float *X, int n, int d, double *sums) {
+  // Each workgroup sums a contiguous chunk of rows; thread t owns feature t % d.
+  const int rowsPer = (n + gridDim.x - 1) / gridDim.x;
+  const int r0 = blockIdx.x * rowsPer, r1 = min(n, r0 + rowsPer);
+  const int lanes = (blockDim.x / d) * d;
+  if ((int)threadIdx.x >= lanes)
+    return;
+  const int f = threadIdx.x % d, group = threadIdx.x / d, groups = lanes / d;
+  double acc = 0.0;
+  for (int r = r0 + group; r < r1; r += groups)
+    acc += (double)X[(size_t)r * d + f];
+  atomicAdd(&sums[f], acc);
+}
+
+__global__ void maxDevKernel(const float *X, int n, int d, const float *mu, unsigned int *maxBits) {
+  float local = 0.0f;
+  const size_t total = (size_t)n * d;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x)
+    local = fmaxf(local, fabsf((float)((double)X[e] - (double)mu[e % d])));
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1)
+    local = fmaxf(local, __shfl_xor(local, m, 64));
+  if ((threadIdx.x & 63) == 0)
+    atomicMax(maxBits, __float_as_uint(local));
+}
+
+__global__ void toHalfKernel(const float *X, int n, int d, int DP, const float *mu, double scale,
+                             _Float16 *xh, float *norm, float *R, float *rho,
+                             unsigned int *maxR, unsigned int *maxRho) {
+  float lr = 0.0f, lrho = 0.0f;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double nn = 0.0, err = 0.0;
+    for (int k = 0; k < DP; ++k) {
+      _Float16 h = (_Float16)0.0f;
+      if (k < d) {
+        const double xs = ((double)X[(size_t)i * d + k] - (double)mu[k]) * scale;
+        h = (_Float16)(float)xs;
+        const double hd = (double)(float)h;
+        nn += hd * hd;
+        err += (hd - xs) * (hd - xs);
+      }
+      xh[(size_t)i * DP + k] = h;
+    }
+    const float nf = (float)nn;
+    norm[i] = nf;
+    const float r = (float)(sqrt(nn) * (1.0 + 1e-6));
+    const float e = (float)(sqrt(err) * (1.0 + 1e-6) + 1e-30);
+    R[i] = r;
+    rho[i] = e;
+    lr = fmaxf(lr, r);
+    lrho = fmaxf(lrho, e);
+  }
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    lr = fmaxf(lr, __shfl_xor(lr, m, 64));
+    lrho = fmaxf(lrho, __shfl_xor(lrho, m, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(maxR, __float_as_uint(lr));
+    atomicMax(maxRho, __float_as_uint(lrho));
+  }
+}
+
+// Permuted copies of the FP16 points and norms; padding columns get zero
+// coordinates and +inf norms (v = +inf, never a candidate).
+__global__ void permuteHalfKernel(const _Float16 *xh, const float *norm, const int *perm, int n,
+                                  int nPad, int DP, _Float16 *xhp, float *normp) {
+  const int vecs = DP / 8;
+  const size_t total = (size_t)nPad * vecs;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int p = (int)(e / vecs), v = (int)(e - (size_t)p * vecs);
+    half8 h;
+    if (p < n) {
+      h = *reinterpret_cast<const half8 *>(xh + (size_t)perm[p] * DP + 8 * v);
+    } else {
+      for (int i = 0; i < 8; ++i)
+        h[i] = (_Float16)0.0f;
+    }
+    *reinterpret_cast<half8 *>(xhp + (size_t)p * DP + 8 * v) = h;
+    if (v == 0)
+      normp[p] = p < n ? norm[perm[p]] : INFINITY;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Candidate pass
+
+struct CandArgs {
+  const _Float16 *xhp; // nPad x DP, permuted
+  const float *normp;  // nPad, permuted
+  const _Float16 *xh;  // n x DP, original order (row operands)
+  const float *norm, *R, *rho; // per point, original order
+  const int *perm;     // position -> original index
+  const int *rows;     // numRows x 3: position, component start, component end
+  int numRows, n, nPad;
+  float Rmax, rhoMax;
+  float gamma;  // (K + 8) 2^-23
+  double kappa; // ((1 + eps)(1 + 2^-21) / (1 - eps)) with eps = (d + 4) 2^-24
+  uint2 *cand;  // numRows x kCandCap: (tile, tile minimum bits)
+  int *candCount;
+  float *theta;
+  unsigned long long *recorded;
+};
+
+__device__ __noinline__ float thetaOf(float m, float Ni, float Ei, float deltai, double sqrtKappa) {
+  if (!(m < INFINITY))
+    return INFINITY;
+  const double A = sqrt(fmax((double)m + (double)Ni + (double)Ei, 0.0)) + (double)deltai;
+  const double t = A * sqrtKappa + (double)deltai;
+  const double th = t * t + (double)Ei - (double)Ni;
+  // Round up to the next float so the FP32 comparison stays conservative.
+  const float f = (float)th;
+  return (double)f >= th ? f : __uint_as_float(__float_as_uint(f) + (f >= 0.0f ? 1u : -1u));
+}
+
+template <int KB>
+__global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
+  constexpr int DP = 16 * KB;
+  constexpr int VEC_PER_TILE = kTileCols * DP / 8; // half8 vectors per column tile
+  __shared__ __align__(16) _Float16 sx[2][kTileCols * DP];
+  __shared__ __align__(16) float sn[2][kTileCols];
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int hi = lane >> 5, l32 = lane & 31;
+  const int rowBase = blockIdx.x * kBlockRows + wave * kWaveRows;
+
+  // Row operands (-2 xh_i), per-row scalars.
+  half8 bop[4][KB];
+  int rowSlot[4], cs[4], ce[4];
+  bool valid[4];
+  float Ni[4], Ei[4], Di[4], runMin[4], th[4];
+  int count[4];
+  const double sqrtKappa = sqrt(a.kappa);
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb) {
+    const int r = rowBase + rb * 32 + l32;
+    rowSlot[rb] = r;
+    valid[rb] = r < a.numRows;
+    int orig = 0;
+    cs[rb] = 0;
+    ce[rb] = 0;
+    if (valid[rb]) {
+      const int pos = a.rows[3 * r];
+      cs[rb] = a.rows[3 * r + 1];
+      ce[rb] = a.rows[3 * r + 2];
+      orig = a.perm[pos];
+    }
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      half8 h;
+      if (valid[rb]) {
+        h = *reinterpret_cast<const half8 *>(a.xh + (size_t)orig * DP + kb * 16 + 8 * hi);
+        h = h * (_Float16)(-2.0f);
+      } else {
+        for (int i = 0; i < 8; ++i)
+          h[i] = (_Float16)0.0f;
+      }
+      bop[rb][kb] = h;
+    }
+    Ni[rb] = valid[rb] ? a.norm[orig] : 0.0f;
+    const float Rs = (valid[rb] ? a.R[orig] : 0.0f) + a.Rmax;
+    Ei[rb] = a.gamma * Rs * Rs * (1.0f + 1e-6f);
+    Di[rb] = (valid[rb] ? a.rho[orig] : 0.0f) + a.rhoMax;
+    runMin[rb] = INFINITY;
+    th[rb] = INFINITY;
+    count[rb] = 0;
+  }
+
+  auto stage = [&](int t, int buf) {
+    const half8 *src = reinterpret_cast<const half8 *>(a.xhp + (size_t)t * kTileCols * DP);
+    half8 *dst = reinterpret_cast<half8 *>(sx[buf]);
+    for (int v = threadIdx.x; v < VEC_PER_TILE; v += 256)
+      dst[v] = src[v];
+    if (threadIdx.x < kTileCols)
+      sn[buf][threadIdx.x] = a.normp[t * kTileCols + threadIdx.x];
+  };
+
+  const int numTiles = a.nPad / kTileCols;
+  stage(0, 0);
+  __syncthreads();
+  for (int t = 0; t < numTiles; ++t) {
+    const int buf = t & 1;
+    const int c0 = t * kTileCols, c1 = c0 + kTileCols;
+    // Wave-uniform decisions about the same-component mask.
+    bool inter = false, contained = true;
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+      if (!valid[rb])
+        continue;
+      inter |= ce[rb] > c0 && cs[rb] < c1;
+      contained &= cs[rb] <= c0 && ce[rb] >= c1;
+    }
+    const bool skipAll = __ballot(!contained) == 0;
+    const bool anyMask = __ballot(inter) != 0;
+    if (!skipAll) {
+      float tmin[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+#pragma unroll 1
+      for (int cb = 0; cb < kTileCols / 32; ++cb) {
+        f32x16 seed;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const float4 s4 =
+              *reinterpret_cast<const float4 *>(&sn[buf][cb * 32 + g * 8 + hi * 4]);
+          seed[4 * g + 0] = s4.x;
+          seed[4 * g + 1] = s4.y;
+          seed[4 * g + 2] = s4.z;
+          seed[4 * g + 3] = s4.w;
+        }
+        half8 aop[KB];
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+          aop[kb] = *reinterpret_cast<const half8 *>(&sx[buf][(cb * 32 + l32) * DP + kb * 16 +
+                                                               8 * hi]);
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+          f32x16 acc = seed;
+#pragma unroll
+          for (int kb = 0; kb < KB; ++kb)
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(aop[kb], bop[rb][kb], acc, 0, 0, 0);
+          if (anyMask) {
+            const unsigned span = (unsigned)(ce[rb] - cs[rb]);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+              const int j = c0 + cb * 32 + mOfReg(r, lane);
+              if ((unsigned)(j - cs[rb]) < span)
+                acc[r] = INFINITY;
+            }
+          }
+          float m = tmin[rb];
+#pragma unroll
+          for (int r = 0; r < 16; r += 2)
+            m = fminf(m, fminf(acc[r], acc[r + 1]));
+          tmin[rb] = m;
+        }
+      }
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) {
+        const float tm = fminf(tmin[rb], __shfl_xor(tmin[rb], 32, 64));
+        if (tm < runMin[rb]) {
+          runMin[rb] = tm;
+          th[rb] = thetaOf(tm, Ni[rb], Ei[rb], Di[rb], sqrtKappa);
+        }
+        if (hi == 0 && valid[rb] && tm < INFINITY && tm <= th[rb]) {
+          if (count[rb] < kCandCap)
+            a.cand[(size_t)rowSlot[rb] * kCandCap + count[rb]] = make_uint2(t, __float_as_uint(tm));
+          ++count[rb];
+        }
+      }
+    }
+    if (t + 1 < numTiles)
+      stage(t + 1, buf ^ 1);
+    __syncthreads();
+  }
+  unsigned long long rec = 0;
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+    if (hi == 0 && valid[rb]) {
+      a.candCount[rowSlot[rb]] = count[rb];
+      a.theta[rowSlot[rb]] = th[rb];
+      rec += count[rb];
+    }
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1)
+    rec += __shfl_xor(rec, m, 64);
+  if (lane == 0 && rec)
+    atomicAdd(a.recorded, rec);
+}
+
+// ---------------------------------------------------------------------------
+// Exact FP32 recheck: one wave per row.
+
+struct RecheckArgs {
+  const float *X;
+  int n, d;
+  const int *perm;
+  const int *rows;
+  int numRows;
+  const uint2 *cand;
+  const int *candCount;
+  const float *theta;
+  bool sqeuclidean;
+  unsigned long long *nearest; // by original index
+  unsigned long long *stats;   // [1] tiles rechecked, [2] pairs, [3] overflow rows
+};
+
+DEV unsigned long long recheckColumn(const RecheckArgs &a, const float *xi, int col, int cs,
+                                     int ce) {
+  if (col >= a.n || (col >= cs && col < ce))
+    return kEmptyKey;
+  const int j = a.perm[col];
+  const float *xj = a.X + (size_t)j * a.d;
+  float acc = 0.0f;
+  for (int k = 0; k < a.d; ++k) {
+    const float diff = xi[k] - xj[k];
+    acc = fmaf(diff, diff, acc);
+  }
+  const float w = a.sqeuclidean ? acc : sqrtf(acc);
+  return w != INFINITY ? packKey(w, (unsigned)j) : kEmptyKey;
+}
+
+__global__ void __launch_bounds__(256) recheckKernel(RecheckArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= a.numRows)
+    return; // wave-uniform
+  const int pos = a.rows[3 * r], cs = a.rows[3 * r + 1], ce = a.rows[3 * r + 2];
+  const int orig = a.perm[pos];
+  const float *xi = a.X + (size_t)orig * a.d;
+  const int count = a.candCount[r];
+  const float th = a.theta[r];
+  unsigned long long best = kEmptyKey;
+  unsigned long long tiles = 0;
+  if (count > kCandCap) {
+    for (int col = lane; col < a.n; col += 64)
+      best = minKey(best, recheckColumn(a, xi, col, cs, ce));
+    tiles = (a.n + kTileCols - 1) / kTileCols;
+  } else {
+    for (int e = 0; e < count; ++e) {
+      const uint2 c = a.cand[(size_t)r * kCandCap + e];
+      if (!(__uint_as_float(c.y) <= th))
+        continue;
+      ++tiles;
+      const int c0 = (int)c.x * kTileCols;
+      best = minKey(best, recheckColumn(a, xi, c0 + lane, cs, ce));
+      best = minKey(best, recheckColumn(a, xi, c0 + 64 + lane, cs, ce));
+    }
+  }
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1)
+    best = minKey(best, shflXorKey(best, m));
+  if (lane == 0) {
+    a.nearest[orig] = best;
+    if (tiles)
+      atomicAdd(&a.stats[1], tiles);
+    atomicAdd(&a.stats[2], tiles * kTileCols);
+    if (count > kCandCap)
+      atomicAdd(&a.stats[3], 1ull);
+  }
+}
+
+int checkLayout() {
+  int state = gLayoutChecked.load();
+  if (state != 0)
+    return state > 0 ? 0 : (int)hipErrorInvalidDeviceFunction;
+  DevArray<int> bad;
+  Status s;
+  s(bad.alloc(1));
+  s(fill(bad, 0, sizeof(int)));
+  if (s.ok()) {
+    hipLaunchKernelGGL(layoutProbeKernel, dim3(1), dim3(64), 0, stream(), bad.get());
+    s((int)hipGetLastError());
+  }
+  int wrong = 0;
+  s(download(&wrong, bad, sizeof(int)));
+  if (!s.ok())
+    return s.code;
+  gLayoutChecked.store(wrong == 0 ? 1 : -1);
+  return wrong == 0 ? 0 : (int)hipErrorInvalidDeviceFunction;
+}
+
+} // namespace
+
+bool mixedPassSupports(int d) { return d >= 1 && d <= 64; }
+
+struct MixedPass {
+  int n = 0, d = 0, DP = 0, nPad = 0;
+  const float *X = nullptr;
+  DevArray<_Float16> xh, xhp;
+  DevArray<float> norm, normp, R, rho, mu, theta;
+  DevArray<int> perm, rows, candCount;
+  DevArray<uint2> cand;
+  DevArray<unsigned long long> stats;
+  float Rmax = 0.0f, rhoMax = 0.0f;
+  std::vector<int> hPerm, hRows, start, cursor;
+};
+
+MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
+  *status = checkLayout();
+  if (*status)
+    return nullptr;
+  MixedPass *mp = new MixedPass;
+  mp->n = n;
+  mp->d = d;
+  mp->DP = (d + 15) / 16 * 16;
+  mp->nPad = (n + kTileCols - 1) / kTileCols * kTileCols;
+  mp->X = X;
+  Status s;
+  DevArray<double> sums;
+  DevArray<unsigned int> maxes;
+  s(sums.alloc(d));
+  s(maxes.alloc(3));
+  s(mp->mu.alloc(d));
+  s(mp->xh.alloc((size_t)n * mp->DP));
+  s(mp->xhp.alloc((size_t)mp->nPad * mp->DP));
+  s(mp->norm.alloc(n));
+  s(mp->normp.alloc(mp->nPad));
+  s(mp->R.alloc(n));
+  s(mp->rho.alloc(n));
+  s(mp->perm.alloc(n));
+  s(mp->rows.alloc((size_t)3 * n));
+  s(mp->candCount.alloc(n));
+  s(mp->theta.alloc(n));
+  s(mp->cand.alloc((size_t)n * kCandCap));
+  s(mp->stats.alloc(4));
+  if (s.ok()) {
+    s(fill(sums, 0, d * sizeof(double)));
+    s(fill(maxes, 0, 3 * sizeof(unsigned int)));
+    s(fill(mp->stats, 0, 4 * sizeof(unsigned long long)));
+    hipLaunchKernelGGL(columnSumsKernel, dim3(std::max(1, std::min(1024, n / 64 + 1))),
+                       dim3(256), 0, stream(), X, n, d, sums.get());
+    std::vector<double> hs(d);
+    s(download(hs.data(), sums, d * sizeof(double)));
+    std::vector<float> hmu(d);
+    for (int k = 0; k < d; ++k)
+      hmu[k] = (float)(hs[k] / n);
+    s(upload(mp->mu, hmu.data(), d * sizeof(float)));
+    hipLaunchKernelGGL(maxDevKernel, dim3(1024), dim3(256), 0, stream(), X, n, d, mp->mu.get(),
+                       maxes.get());
+    unsigned int mb = 0;
+    s(download(&mb, maxes, sizeof(mb)));
+    const float maxDev = floatOf(mb);
+    // Power-of-two scale: max |x'| in (2^13, 2^14].
+    int e = 0;
+    (void)std::frexp(maxDev > 0.0f && std::isfinite(maxDev) ? maxDev : 1.0f, &e);
+    const double scale = std::ldexp(1.0, 14 - e);
+    hipLaunchKernelGGL(toHalfKernel, dim3(1024), dim3(256), 0, stream(), X, n, d, mp->DP,
+                       mp->mu.get(), scale, mp->xh.get(), mp->norm.get(), mp->R.get(),
+                       mp->rho.get(), maxes.get() + 1, maxes.get() + 2);
+    unsigned int rr[2] = {0, 0};
+    s(download(rr, maxes.get() + 1, sizeof(rr)));
+    mp->Rmax = floatOf(rr[0]);
+    mp->rhoMax = floatOf(rr[1]);
+  }
+  *status = s.code;
+  if (!s.ok()) {
+    delete mp;
+    return nullptr;
+  }
+  mp->hPerm.resize(n);
+  mp->hRows.resize((size_t)3 * n);
+  return mp;
+}
+
+void mixedPassDestroy(MixedPass *mp) {
+  if (!mp)
+    return;
+  unsigned long long st[4] = {0, 0, 0, 0};
+  if (download(st, mp->stats, sizeof(st)) == 0)
+    for (int i = 0; i < 4; ++i)
+      __atomic_fetch_add(&gStats[i], st[i], __ATOMIC_RELAXED);
+  delete mp;
+}
+
+int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeuclidean,
+                 const unsigned long long *prev, unsigned long long *nearest) {
+  const int n = mp->n;
+  // Component-contiguous order (by component, then original index).
+  mp->start.assign(numComponents + 1, 0);
+  for (int i = 0; i < n; ++i)
+    ++mp->start[comp[i] + 1];
+  for (int c = 0; c < numComponents; ++c)
+    mp->start[c + 1] += mp->start[c];
+  mp->cursor.assign(mp->start.begin(), mp->start.end() - 1);
+  for (int i = 0; i < n; ++i)
+    mp->hPerm[mp->cursor[comp[i]]++] = i;
+  // Rows to search: all in the first round; later only points whose previous
+  // nearest joined their own component.
+  int numRows = 0;
+  for (int p = 0; p < n; ++p) {
+    const int i = mp->hPerm[p];
+    const bool stale = prev == nullptr || prev[i] == kEmptyKey || comp[keyIndex(prev[i])] == comp[i];
+    if (!stale)
+      continue;
+    mp->hRows[3 * numRows] = p;
+    mp->hRows[3 * numRows + 1] = mp->start[comp[i]];
+    mp->hRows[3 * numRows + 2] = mp->start[comp[i] + 1];
+    ++numRows;
+  }
+  Status s;
+  s((int)hipMemcpyAsync(mp->perm, mp->hPerm.data(), n * sizeof(int), hipMemcpyHostToDevice,
+                        stream()));
+  s((int)hipMemcpyAsync(mp->rows, mp->hRows.data(), (size_t)3 * numRows * sizeof(int),
+                        hipMemcpyHostToDevice, stream()));
+  if (!s.ok() || numRows == 0)
+    return s.code ? s.code : sync();
+  hipLaunchKernelGGL(permuteHalfKernel, dim3(2048), dim3(256), 0, stream(), mp->xh.get(),
+                     mp->norm.get(), mp->perm.get(), n, mp->nPad, mp->DP, mp->xhp.get(),
+                     mp->normp.get());
+  CandArgs ca;
+  ca.xhp = mp->xhp;
+  ca.normp = mp->normp;
+  ca.xh = mp->xh;
+  ca.norm = mp->norm;
+  ca.R = mp->R;
+  ca.rho = mp->rho;
+  ca.perm = mp->perm;
+  ca.rows = mp->rows;
+  ca.numRows = numRows;
+  ca.n = n;
+  ca.nPad = mp->nPad;
+  ca.Rmax = mp->Rmax;
+  ca.rhoMax = mp->rhoMax;
+  const int KB = mp->DP / 16;
+  ca.gamma = (float)((16 * KB + 8) * std::ldexp(1.0, -23));
+  const double eps = (mp->d + 4) * std::ldexp(1.0, -24);
+  ca.kappa = (1.0 + eps) * (1.0 + std::ldexp(1.0, -21)) / (1.0 - eps) * (1.0 + 1e-9);
+  ca.cand = mp->cand;
+  ca.candCount = mp->candCount;
+  ca.theta = mp->theta;
+  ca.recorded = mp->stats.get();
+  const int blocks = (numRows + kBlockRows - 1) / kBlockRows;
+  {
+    PhaseTimer timer(kPhaseMfmaCandidates);
+    addUnits(kPhaseMfmaCandidates, (long long)numRows * n);
+    switch (KB) {
+    case 1: hipLaunchKernelGGL(mfmaCandidateKernel<1>, dim3(blocks), dim3(256), 0, stream(), ca); break;
+    case 2: hipLaunchKernelGGL(mfmaCandidateKernel<2>, dim3(blocks), dim3(256), 0, stream(), ca); break;
+    case 3: hipLaunchKernelGGL(mfmaCandidateKernel<3>, dim3(blocks), dim3(256), 0, stream(), ca); break;
+    default: hipLaunchKernelGGL(mfmaCandidateKernel<4>, dim3(blocks), dim3(256), 0, stream(), ca); break;
+    }
+    s((int)hipGetLastError());
+  }
+  RecheckArgs ra;
+  ra.X = mp->X;
+  ra.n = n;
+  ra.d = mp->d;
+  ra.perm = mp->perm;
+  ra.rows = mp->rows;
+  ra.numRows = numRows;
+  ra.cand = mp->cand;
+  ra.candCount = mp->candCount;
+  ra.theta = mp->theta;
+  ra.sqeuclidean = sqeuclidean;
+  ra.nearest = nearest;
+  ra.stats = mp->stats;
+  {
+    PhaseTimer timer(kPhaseRecheck);
+    hipLaunchKernelGGL(recheckKernel, dim3((numRows + 3) / 4), dim3(256), 0, stream(), ra);
+    s((int)hipGetLastError());
+  }
+  return s.code;
+}
+
+} // namespace cuhc
+
+extern "C" void cuhc_mixed_stats(unsigned long long *recorded, unsigned long long *rechecked,
+                                 unsigned long long *pairs, unsigned long long *overflows) {
+  using cuhc::gStats;
+  *recorded = __atomic_exchange_n(&gStats[0], 0ull, __ATOMIC_RELAXED);
+  *rechecked = __atomic_exchange_n(&gStats[1], 0ull, __ATOMIC_RELAXED);
+  *pairs = __atomic_exchange_n(&gStats[2], 0ull, __ATOMIC_RELAXED);
+  *overflows = __atomic_exchange_n(&gStats[3], 0ull, __ATOMIC_RELAXED);
+}

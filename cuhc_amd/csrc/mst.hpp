@@ -1,0 +1,69 @@
+// Boruvka minimum spanning tree and the linkage/HDBSCAN drivers built on it.
+#pragma once
+
+#include "host_tree.hpp"
+#include "passes.hpp"
+
+#include <vector>
+
+namespace cuhc {
+
+// Points on the device, possibly in spatial order (order[i] = original index
+// of position i; empty = identity).
+struct DevicePoints {
+  DevArray<float> points;
+  DevArray<int> orig; // device copy of order (only when ordered)
+  std::vector<int> order;
+  bool ordered() const { return !order.empty(); }
+  // Uploads hostPoints (n x d), in spatial order if spatial is set.
+  int upload(const float *hostPoints, int n, int d, bool spatial);
+  // Adopts device-resident points (no copy unless spatial order is wanted).
+  int adopt(const float *devicePoints, int n, int d, bool spatial);
+  const float *data() const { return external ? external : points.get(); }
+  const float *external = nullptr;
+};
+
+struct MstOptions {
+  bool prune = false;          // points in spatial order; use the walk pass
+  const int *hostOrig = nullptr; // position -> original index (host), null = identity
+  const int *devOrig = nullptr;  // the same on the device
+  PassStats *stats = nullptr;
+  bool mixed = false;          // FP16 MFMA candidate pass (plain euclidean weights only)
+};
+
+// Exact MST of the complete graph over the points under the total order
+// (weight, smaller original index, larger original index). Edges come back in
+// original indices. Returns 0, a HIP error, or kErrDisconnected.
+int boruvkaMst(const float *X, int n, int d, MetricSpec m, WeightSpec w, const MstOptions &opt,
+               std::vector<Edge> &edges);
+
+// Core distances: core[i] = k-th smallest distance from point i (itself = 1st).
+enum CoreMethod : int { kCoreAuto = 0, kCoreFused = 1, kCoreRadix = 2 };
+constexpr int kMaxFusedK = 64;
+int coreDistances(const float *X, int n, int d, MetricSpec m, int k, float *core, int method,
+                  bool prune);
+
+// The single-linkage tree of mutual reachability (host tree, host core in
+// original order).
+int hdbscanTree(const DevicePoints &pts, int n, int d, MetricSpec m, int minSamples, float alpha,
+                int coreMethod, bool prune, PassStats *stats, double *tree, float *hostCore);
+
+// Plain single linkage through Boruvka.
+int singleLinkageMst(const DevicePoints &pts, int n, int d, MetricSpec m, bool prune, bool mixed,
+                     PassStats *stats, double *tree);
+
+// FP16 MFMA certified-candidate pass (mfma.hip): for every point of X (n x d,
+// original order), the exact nearest point in another component under the
+// FP32 euclidean (or squared euclidean) distance, as packKey(weight, j), the
+// same value the FP32 tiles give. hostComp[] are component labels by point;
+// hostPrev (null in the first round) is the previous round's result: a point
+// whose previous nearest is still in another component keeps it (the set of
+// other-component points only shrinks), so only the other rows are searched.
+struct MixedPass;
+MixedPass *mixedPassCreate(const float *X, int n, int d, int *status);
+void mixedPassDestroy(MixedPass *mp);
+int mixedPassRun(MixedPass *mp, const int *hostComp, int numComponents, bool sqeuclidean,
+                 const unsigned long long *hostPrev, unsigned long long *nearestDevice);
+bool mixedPassSupports(int d);
+
+} // namespace cuhc

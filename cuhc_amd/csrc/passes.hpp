@@ -1,0 +1,79 @@
+// Host entry points of the all-pairs kernels (distance matrices, nearest
+// other component passes, spatial pruning). All pointers are device pointers
+// unless noted; everything runs on cuhc::stream().
+#pragma once
+
+#include "common.hpp"
+#include "runtime.hpp"
+
+#include <vector>
+
+namespace cuhc {
+
+struct MetricSpec {
+  int id;
+  float p; // Minkowski order
+};
+
+enum WeightKind : int { kWeightPlain = 0, kWeightMReach = 1, kWeightWard = 2 };
+
+struct WeightSpec {
+  int kind = kWeightPlain;
+  const float *core = nullptr; // device, for kWeightMReach
+  float alpha = 1.0f;
+  const int *size = nullptr; // device, for kWeightWard
+};
+
+bool validMetric(int id);
+bool metricHasBoxBound(int id);
+int tilePoints(int metricId); // tile side of the metric's all-pairs kernels (128 or 64)
+
+// out[i * nB + j] = metric(A[i], B[j]).
+int pairwiseMatrix(const float *A, int nA, const float *B, int nB, int d, MetricSpec m,
+                   float *out);
+
+// Condensed upper triangle: out[i*n - i*(i+1)/2 + (j - i - 1)] = metric(X[i], X[j]), i < j.
+int condensedMatrix(const float *X, int n, int d, MetricSpec m, float *out);
+
+// nearest[i] = min over j with comp[j] != comp[i] of packKey(weight(i, j), id(j)),
+// combined by atomicMin, so nearest[] must start at kEmptyKey. Each unordered
+// pair is computed once (tiles I <= J). id(j) = orig[j] (device) for points in
+// spatial order, else j (orig == nullptr).
+int nearestOtherSymmetric(const float *X, int n, int d, MetricSpec m, WeightSpec w,
+                          const int *comp, const int *orig, unsigned long long *nearest);
+
+// ---------------------------------------------------------------------------
+// Spatial pruning (reference src/spatial.cuh)
+
+constexpr int kPruneMinPoints = 16384;
+constexpr int kPruneMaxFeatures = 64;
+constexpr int kPruneHardMaxFeatures = 256;
+constexpr double kPruneGiveUpShare = 0.5;
+enum PruneMode : int { kPruneAuto = 0, kPruneAlways = 1, kPruneNever = 2 };
+
+bool shouldPrune(int mode, int metricId, int n, int d);
+
+// k-d order on the host: order[i] = original index of the point at position i.
+std::vector<int> spatialOrder(const float *hostPoints, int n, int d);
+
+// dst[i] = src[order[i]] (rows of d floats).
+int gatherRows(const float *src, const int *order, int n, int d, float *dst);
+
+struct PassStats {
+  unsigned long long tilesComputed = 0, tilesTotal = 0;
+};
+
+// Pruned pass for points in spatial order: same contract as
+// nearestOtherSymmetric, except that keys carry orig[j] (device array), so ties
+// break by original index exactly as in an unpruned run. Returns the share of
+// tile pairs computed in *share. rowTileStride > 1 runs every stride-th row
+// tile only (sampling).
+int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, const int *comp,
+                     const int *orig, unsigned long long *nearest, double *share,
+                     PassStats *stats, int rowTileStride = 1);
+
+// Share of tiles a pruned pass would compute, from a sample of row tiles
+// (Boruvka's first pass: plain weights, every point its own component).
+int estimatePrunedShare(const float *X, int n, int d, MetricSpec m, double *share);
+
+} // namespace cuhc

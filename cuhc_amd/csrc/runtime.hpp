@@ -1,0 +1,90 @@
+// Host-side runtime shared by the entry points: the library's HIP stream,
+// device buffers, first-error status, and per-phase kernel timing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <vector>
+
+namespace cuhc {
+
+// The one stream every kernel and copy of the library runs on (created on
+// first use, default priority, non-blocking with respect to the null stream).
+hipStream_t stream();
+
+// Number of compute units of device 0 (cached).
+int numCUs();
+
+// Keeps the first non-zero status of a sequence of calls.
+struct Status {
+  int code = 0;
+  void operator()(int result) {
+    if (code == 0)
+      code = result;
+  }
+  bool ok() const { return code == 0; }
+};
+
+// Owning device allocation.
+template <class T> struct DevArray {
+  T *p = nullptr;
+  size_t count = 0;
+  DevArray() = default;
+  DevArray(const DevArray &) = delete;
+  DevArray &operator=(const DevArray &) = delete;
+  ~DevArray() { release(); }
+  int alloc(size_t n) {
+    release();
+    count = n;
+    if (n == 0)
+      return 0;
+    return (int)hipMalloc(&p, n * sizeof(T));
+  }
+  void release() {
+    if (p)
+      (void)hipFree(p);
+    p = nullptr;
+    count = 0;
+  }
+  T *get() const { return p; }
+  operator T *() const { return p; }
+  size_t bytes() const { return count * sizeof(T); }
+};
+
+// Synchronous helpers on the library stream.
+int upload(void *dst, const void *src, size_t bytes);
+int download(void *dst, const void *src, size_t bytes);
+int fill(void *dst, int byte, size_t bytes);
+int sync();
+
+// Per-phase GPU timing with HIP events on the library stream. Off by default;
+// cuhc_set_timing(1) turns it on. A phase is bracketed by begin()/end(); the
+// elapsed time is collected at the next collect() (after a stream sync).
+enum Phase : int {
+  kPhaseDistanceTiles = 0,  // FP32 direct-form all-pairs passes (Boruvka, Ward)
+  kPhaseMfmaCandidates = 1, // FP16 MFMA candidate pass
+  kPhaseRecheck = 2,        // FP32 recheck of certified candidates
+  kPhaseCore = 3,           // core distances
+  kPhaseContract = 4,       // component bookkeeping between rounds
+  kPhaseDense = 5,          // dense Lance-Williams loop
+  kPhaseRnn = 6,            // RNN merge/compaction kernels
+  kPhaseMlp = 7,            // learned-radius network
+  kNumPhases = 8,
+};
+
+struct PhaseTimer {
+  int phase;
+  bool active;
+  explicit PhaseTimer(int phase);
+  ~PhaseTimer();
+};
+
+bool timingEnabled();
+void setTiming(bool on);
+void collectTimings(); // syncs the stream and folds finished events into totals
+void readTimings(double *ms, long long *launches, long long *units, int count);
+void addUnits(int phase, long long units); // algorithmic work units (pairs) per phase
+void resetTimings();
+
+} // namespace cuhc

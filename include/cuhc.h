@@ -1,0 +1,110 @@
+/* C ABI of libcuhc.so, the MI355X (gfx950) build of cuhc's clustering path.
+ *
+ * The first eight functions are the drop-in boundary: the same names,
+ * signatures, argument meanings, ID tables and status codes as the reference
+ * library's extern "C" interface (reference src/capi.cu). Host pointers in and
+ * out, blocking calls, one GPU (device 0). The functions after them have no
+ * counterpart in the reference; each says what it is for.
+ *
+ * Status codes: 0 = success; a positive value is a hipError_t (2 = out of
+ * memory); -1 invalid metric, -2 invalid shape, -3 invalid linkage,
+ * -4 invalid algorithm (or core method) for the request, -5 disconnected (some
+ * points have no finite distance to the rest), -6 non-finite merge heights.
+ *
+ * ID tables. metric: 0 euclidean, 1 sqeuclidean, 2 manhattan, 3 chebyshev,
+ * 4 minkowski (metricParam = p), 5 cosine. linkage: 0 single, 1 complete,
+ * 2 average, 3 ward. algorithm: 0 dense, 1 mst (single only), 2 nn_chain and
+ * 3 rnn (ward + euclidean only). core method: 0 auto, 1 fused (k <= 64),
+ * 2 radix. prune mode: 0 auto, 1 always, 2 never.
+ */
+#ifndef CUHC_H
+#define CUHC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* distances (nA x nB, row-major) = metric(A[i], B[j]); B == NULL means B = A.
+ * Replaces reference src/capi.cu:138-173 (cuhc_pairwise_distances). */
+int cuhc_pairwise_distances(const float *A, int nA, const float *B, int nB, int d, int metricId,
+                            float metricParam, float *distances);
+
+/* Full merge tree in scipy's linkage format: tree receives (n - 1) x 4
+ * doubles {id1, id2, height, size}. Replaces reference src/capi.cu:179-257. */
+int cuhc_linkage(const float *X, int n, int d, int metricId, float metricParam, int linkageId,
+                 int algorithmId, double *tree);
+
+/* Ward through RNN rounds with explicit hand-over settings (test hook).
+ * Replaces reference src/capi.cu:261-279. */
+int cuhc_ward_rnn_with_options(const float *X, int n, int d, float minProgress, int denseBelow,
+                               int allowDense, double *tree);
+
+/* HDBSCAN's single-linkage tree of mutual reachability; core (n floats, may be
+ * NULL) receives the core distances. Replaces reference src/capi.cu:285-309. */
+int cuhc_hdbscan_tree(const float *X, int n, int d, int metricId, float metricParam,
+                      int minSamples, float alpha, int coreMethod, double *tree, float *core);
+
+/* Condensed tree from an (n - 1) x 4 single-linkage tree: writes up to
+ * capacity rows and returns the row count, or -(rows needed).
+ * Replaces reference src/capi.cu:314-327. */
+long long cuhc_condense_tree(const double *tree, long long n, long long minClusterSize,
+                             long long capacity, long long *parent, long long *child,
+                             double *lambda, long long *size);
+
+/* Spatial pruning mode for mst, rnn and HDBSCAN (results never change).
+ * Replaces reference src/capi.cu:332-337. */
+int cuhc_set_pruning(int mode);
+
+/* Tiles computed / total by pruned passes since the last call, then resets.
+ * Replaces reference src/capi.cu:341-345. */
+void cuhc_prune_stats(unsigned long long *computed, unsigned long long *total);
+
+/* Message for a status code. Replaces reference src/capi.cu:347-364. */
+const char *cuhc_error_string(int code);
+
+/* ---- Extensions (no counterpart in the reference) ---------------------- */
+
+/* Precision of the single-linkage Borůvka passes: 0 auto (FP16 MFMA
+ * candidates + certified FP32 recheck for euclidean/sqeuclidean, d >= 16),
+ * 1 plain FP32 tiles, 2 mixed whenever the metric allows. The tree is
+ * identical in every mode. For benchmarks and tests. */
+int cuhc_set_precision(int mode);
+
+/* cuhc_linkage with X already in device memory (n x d floats, row-major, from
+ * hipMalloc or torch). tree is host memory. For callers that keep points
+ * resident in HBM (zero-copy input). */
+int cuhc_linkage_device(const float *deviceX, int n, int d, int metricId, float metricParam,
+                        int linkageId, int algorithmId, double *tree);
+
+/* cuhc_hdbscan_tree with X in device memory; tree and core are host memory. */
+int cuhc_hdbscan_tree_device(const float *deviceX, int n, int d, int metricId, float metricParam,
+                             int minSamples, float alpha, int coreMethod, double *tree,
+                             float *core);
+
+/* Device buffers on the library's HIP runtime, for callers that keep inputs
+ * resident in HBM across calls (the *_device entry points). Return a status. */
+int cuhc_device_alloc(unsigned long long bytes, void **ptr);
+int cuhc_device_free(void *ptr);
+int cuhc_copy_to_device(void *dst, const void *src, unsigned long long bytes);
+int cuhc_copy_to_host(void *dst, const void *src, unsigned long long bytes);
+
+/* Per-phase GPU timing with HIP events on the library stream (0 = off).
+ * Phases: 0 FP32 distance tiles, 1 FP16 MFMA candidates, 2 FP32 recheck,
+ * 3 core distances, 4 component bookkeeping, 5 dense loop, 6 RNN kernels,
+ * 7 learned-radius network. */
+int cuhc_set_timing(int on);
+/* Fills up to count phases: total ms, launches, algorithmic units (pairs). */
+void cuhc_read_timings(double *ms, long long *launches, long long *units, int count);
+void cuhc_reset_timings(void);
+
+/* Counters of the mixed-precision pass since the last call (then resets):
+ * candidate tiles recorded, tiles rechecked in FP32, pairs rechecked,
+ * rows whose candidate list overflowed (rechecked in full). */
+void cuhc_mixed_stats(unsigned long long *recorded, unsigned long long *rechecked,
+                      unsigned long long *pairs, unsigned long long *overflows);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUHC_H */

@@ -1,0 +1,143 @@
+"""Single linkage through Boruvka on the GPU: plain FP32 tiles, the pruned walk,
+and the FP16-MFMA certified-candidate form all give the oracle's tree bit for
+bit (reference tests/test_mst.py, tests/test_spatial_pruning.py)."""
+
+import numpy as np
+import pytest
+from scipy.cluster.hierarchy import linkage as scipy_linkage
+
+import cuhc_amd as cuhc
+import oracle
+from _helpers import assert_valid_tree, blobs, max_rel_height_error, tree_clusters
+from cuhc_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def run(X, metric="euclidean", precision="fp32", pruning="never", p=2.0):
+    _lib.set_precision(precision)
+    _lib.set_pruning(pruning)
+    try:
+        return cuhc.linkage(X, method="single", metric=metric, p=p, algorithm="mst")
+    finally:
+        _lib.set_precision("auto")
+        _lib.set_pruning("auto")
+
+
+@pytest.mark.parametrize("metric", ["euclidean", "sqeuclidean", "manhattan", "chebyshev",
+                                    "cosine"])
+@pytest.mark.parametrize("n", [2, 3, 127, 128, 129, 300])
+def test_plain_matches_oracle(metric, n):
+    X = blobs(n_samples=n, seed=n)
+    np.testing.assert_array_equal(run(X, metric), oracle.single_linkage(X, metric))
+
+
+@pytest.mark.parametrize("n, d", [(2, 32), (129, 16), (300, 32), (1000, 20), (2500, 64),
+                                  (3000, 32)])
+@pytest.mark.parametrize("metric", ["euclidean", "sqeuclidean"])
+def test_mixed_matches_oracle(metric, n, d):
+    X = np.random.default_rng(n + d).standard_normal((n, d), dtype=np.float32)
+    ours = run(X, metric, precision="mixed")
+    np.testing.assert_array_equal(ours, oracle.single_linkage(X, metric))
+
+
+def test_mixed_on_clustered_and_offset_data():
+    # Far from the origin and strongly clustered: the FP16 bound must still
+    # cover every candidate (centring and scaling are per dataset).
+    X = blobs(n_samples=2000, n_features=24, centers=7, seed=5) + np.float32(1000.0)
+    np.testing.assert_array_equal(run(X, precision="mixed"), oracle.single_linkage(X))
+    _lib.mixed_stats()
+    Y = np.repeat(np.random.default_rng(4).normal(size=(300, 16)), 3, axis=0).astype(np.float32)
+    np.testing.assert_array_equal(run(Y, precision="mixed"), run(Y, precision="fp32"))
+
+
+def test_mixed_on_tied_grid():
+    g = np.stack(np.meshgrid(np.arange(20), np.arange(20)), axis=-1).reshape(-1, 2)
+    X = np.hstack([g, np.zeros((400, 14))]).astype(np.float32)
+    Z = run(X, precision="mixed")
+    np.testing.assert_array_equal(Z, oracle.single_linkage(X))
+    np.testing.assert_array_equal(Z[:, 2], 1.0)
+
+
+@pytest.mark.parametrize("n_features", [1, 2, 3, 8, 20])
+@pytest.mark.parametrize("metric", ["euclidean", "sqeuclidean", "manhattan", "chebyshev"])
+def test_pruned_equals_unpruned(metric, n_features):
+    X = blobs(n_samples=3000, n_features=n_features, centers=30, seed=n_features)
+    pruned = run(X, metric, pruning="always")
+    full = run(X, metric, pruning="never")
+    np.testing.assert_array_equal(pruned, full)
+    np.testing.assert_array_equal(full, oracle.single_linkage(X, metric))
+
+
+def test_pruning_skips_tiles_on_clustered_data():
+    X = blobs(n_samples=20000, n_features=2, centers=30, seed=30)
+    _lib.prune_stats()
+    run(X, pruning="always")
+    computed, total = _lib.prune_stats()
+    assert total > 0 and computed < 0.2 * total
+
+
+def test_grid_ties_and_sizes_off_tiles():
+    g = np.stack(np.meshgrid(np.arange(20), np.arange(20)), axis=-1).reshape(-1, 2)
+    X = g.astype(np.float32)
+    Z = run(X)
+    np.testing.assert_array_equal(Z, oracle.single_linkage(X))
+    np.testing.assert_array_equal(Z[:, 2], 1.0)
+    for n in [129, 255, 257, 1000]:
+        Y = blobs(n_samples=n, n_features=3, seed=n)
+        np.testing.assert_array_equal(run(Y, pruning="always"), oracle.single_linkage(Y))
+
+
+def test_c1_recipe_against_oracle_and_scipy():
+    # C1: n = 20,000, d = 32, N(0,1) float32, seed 0. Bit-identical to the oracle
+    # in both precisions; the same dendrogram as scipy in float64 with the
+    # largest relative height error reported.
+    X = np.random.default_rng(0).standard_normal((20_000, 32), dtype=np.float32)
+    ref = oracle.single_linkage(X)
+    plain = run(X, precision="fp32")
+    mixed = run(X, precision="mixed")
+    np.testing.assert_array_equal(plain, ref)
+    np.testing.assert_array_equal(mixed, ref)
+    # Against scipy in float64: float32 rounding makes some heights tie (or
+    # swap order) that float64 separates, which renumbers a few clusters; count
+    # them and bound the height error on all clusters the trees share.
+    sp = scipy_linkage(X.astype(np.float64), "single")
+    ours, ref = tree_clusters(mixed), tree_clusters(sp)
+    swapped = len(ours.keys() - ref.keys())
+    err = max_rel_height_error(mixed, sp)
+    print(f"C1 vs scipy float64: {swapped} of {len(ref)} clusters differ by tie order; "
+          f"max relative height error {err:.3e}")
+    assert swapped <= 0.001 * len(ref)
+    assert err < 1e-5
+
+
+def test_auto_and_estimator():
+    X = blobs(n_samples=500, seed=14)
+    np.testing.assert_array_equal(cuhc.linkage(X), run(X))
+    labels = cuhc.AgglomerativeClustering(n_clusters=6, linkage="single").fit(X).labels_
+    assert len(set(labels.tolist())) == 6
+    assert_valid_tree(cuhc.linkage(X), 500)
+
+
+def test_device_entry_point_matches_host_entry():
+    X = np.random.default_rng(9).standard_normal((3000, 32), dtype=np.float32)
+    dev = _lib.DeviceArray(X)
+    tree = np.empty((2999, 4), dtype=np.float64)
+    for mode in ["fp32", "mixed"]:
+        _lib.set_precision(mode)
+        try:
+            _lib.check(_lib.lib.cuhc_linkage_device(dev.ptr, 3000, 32, 0, 2.0, 0, 1,
+                                                    _lib.double_ptr(tree)))
+        finally:
+            _lib.set_precision("auto")
+        np.testing.assert_array_equal(tree, oracle.single_linkage(X))
+    H = np.ascontiguousarray(X[:, :8])
+    hdev = _lib.DeviceArray(H)
+    core = np.empty(3000, dtype=np.float32)
+    _lib.check(_lib.lib.cuhc_hdbscan_tree_device(hdev.ptr, 3000, 8, 0, 2.0, 5, 1.0, 0,
+                                                 _lib.double_ptr(tree), _lib.float_ptr(core)))
+    Zo, co = oracle.hdbscan_tree(H, 5)
+    np.testing.assert_array_equal(core, co)
+    np.testing.assert_array_equal(tree, Zo)
+    dev.free()
+    hdev.free()
