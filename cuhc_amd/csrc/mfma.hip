@@ -29,11 +29,16 @@
 // inside every row's own component are skipped, tiles crossing one are
 // masked per value, all others need no mask. Each wave owns 128 rows (4 row
 // blocks of 32, the MFMA N dimension); the workgroup's 4 waves share column
-// tiles of 128 points staged in LDS (double-buffered). Per 128-column tile and
-// row the kernel records (tile, tile minimum) when the tile minimum is within
-// theta; the recheck kernel recomputes the FP32 direct-form weights of the
-// recorded tiles still within the final theta and keeps the exact argmin
-// (weight, original index). Rows whose list overflows are rechecked in full.
+// tiles of 128 points staged in LDS (double-buffered, the next tile's global
+// loads in flight while the current one is computed). A lane holds 16 of the
+// 32 columns of each MFMA block for its row; it records (half-block, minimum)
+// in its own list when the minimum is within its theta (the two halves of a
+// row share their minimum once per tile, so theta stays tight). The recheck
+// kernel recomputes the FP32 direct-form weights of the recorded half-blocks
+// still within the row's final theta and keeps the exact argmin (weight,
+// original index). Rows whose list overflows are rechecked in full. This
+// translation unit is compiled with -fno-honor-nans (inputs are validated
+// finite), which lets fminf map to v_min/v_min3 without canonicalisation.
 #include "mst.hpp"
 
 #include <algorithm>
@@ -183,9 +188,9 @@ struct CandArgs {
   float Rmax, rhoMax;
   float gamma;  // (K + 8) 2^-23
   double kappa; // ((1 + eps)(1 + 2^-21) / (1 - eps)) with eps = (d + 4) 2^-24
-  uint2 *cand;  // numRows x kCandCap: (tile, tile minimum bits)
-  int *candCount;
-  float *theta;
+  uint2 *cand;  // (numRows x 2 halves) x kCandCap: (half-block, block minimum bits)
+  int *candCount; // numRows x 2
+  float *theta;   // numRows
   unsigned long long *recorded;
 };
 
@@ -200,42 +205,111 @@ __device__ __noinline__ float thetaOf(float m, float Ni, float Ei, float deltai,
   return (double)f >= th ? f : __uint_as_float(__float_as_uint(f) + (f >= 0.0f ? 1u : -1u));
 }
 
+// Operands of one 32-column block: A (column points) and the C seed (their norms).
+template <int KB> struct BlockOps {
+  half8 a[KB];
+  f32x16 seed;
+};
+
+template <int KB>
+DEV void loadBlock(BlockOps<KB> &o, const _Float16 *sx, const float *sn, int cb, int l32, int hi) {
+  constexpr int DP = 16 * KB;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const float4 s4 = *reinterpret_cast<const float4 *>(&sn[cb * 32 + g * 8 + hi * 4]);
+    o.seed[4 * g + 0] = s4.x;
+    o.seed[4 * g + 1] = s4.y;
+    o.seed[4 * g + 2] = s4.z;
+    o.seed[4 * g + 3] = s4.w;
+  }
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb)
+    o.a[kb] = *reinterpret_cast<const half8 *>(&sx[(cb * 32 + l32) * DP + kb * 16 + 8 * hi]);
+}
+
+// Per-lane candidate state of one row block (this lane's half of the row's columns).
+struct RowLane {
+  int slot;      // row slot, or -1
+  int cs, ce;    // the row's component: permuted positions [cs, ce)
+  float N, E, D; // norm, accumulation bound, rounding radius
+  float runMin;  // this lane's running minimum of v
+  float rowMin;  // the row's minimum over both halves at the last tile end
+  float th;      // theta of rowMin
+  int count;     // entries recorded in this lane's list
+};
+
+// v for one row block: C seeded with the column norms, A the column points,
+// B this row block's -2 xh.
+template <int KB>
+DEV f32x16 mfmaBlock(const BlockOps<KB> &o, const half8 (&bop)[KB]) {
+  f32x16 acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(o.a[0], bop[0], o.seed, 0, 0, 0);
+#pragma unroll
+  for (int kb = 1; kb < KB; ++kb)
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(o.a[kb], bop[kb], acc, 0, 0, 0);
+  return acc;
+}
+
+template <int KB>
+DEV void blockEpilogue(f32x16 acc, RowLane &rl, bool anyMask, int j0, int lane, int block,
+                       const CandArgs &a, int hi) {
+  if (anyMask) {
+    const unsigned span = (unsigned)(rl.ce - rl.cs);
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      if ((unsigned)(j0 + mOfReg(r, lane) - rl.cs) < span)
+        acc[r] = INFINITY;
+  }
+  float m = fminf(acc[0], acc[1]);
+#pragma unroll
+  for (int r = 2; r < 16; r += 2)
+    m = fminf(m, fminf(acc[r], acc[r + 1]));
+  rl.runMin = fminf(rl.runMin, m);
+  // Record this lane's 16 columns of the block when they can hold a candidate
+  // (the lane's theta is never below the row's final one, so nothing needed
+  // is missed).
+  if (rl.slot >= 0 && m <= rl.th && m < INFINITY) {
+    if (rl.count < kCandCap)
+      a.cand[((size_t)rl.slot * 2 + hi) * kCandCap + rl.count] =
+          make_uint2((unsigned)block, __float_as_uint(m));
+    ++rl.count;
+  }
+}
+
 template <int KB>
 __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
   constexpr int DP = 16 * KB;
   constexpr int VEC_PER_TILE = kTileCols * DP / 8; // half8 vectors per column tile
+  constexpr int VPT = VEC_PER_TILE / 256;          // per thread
+  static_assert(VEC_PER_TILE % 256 == 0, "tile staging");
   __shared__ __align__(16) _Float16 sx[2][kTileCols * DP];
   __shared__ __align__(16) float sn[2][kTileCols];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int hi = lane >> 5, l32 = lane & 31;
   const int rowBase = blockIdx.x * kBlockRows + wave * kWaveRows;
-
-  // Row operands (-2 xh_i), per-row scalars.
-  half8 bop[4][KB];
-  int rowSlot[4], cs[4], ce[4];
-  bool valid[4];
-  float Ni[4], Ei[4], Di[4], runMin[4], th[4];
-  int count[4];
   const double sqrtKappa = sqrt(a.kappa);
+
+  // Row operands (-2 xh_i) and per-row state.
+  half8 bop[4][KB];
+  RowLane rl[4];
 #pragma unroll
   for (int rb = 0; rb < 4; ++rb) {
     const int r = rowBase + rb * 32 + l32;
-    rowSlot[rb] = r;
-    valid[rb] = r < a.numRows;
+    const bool valid = r < a.numRows;
     int orig = 0;
-    cs[rb] = 0;
-    ce[rb] = 0;
-    if (valid[rb]) {
+    rl[rb].slot = valid ? r : -1;
+    rl[rb].cs = 0;
+    rl[rb].ce = 0;
+    if (valid) {
       const int pos = a.rows[3 * r];
-      cs[rb] = a.rows[3 * r + 1];
-      ce[rb] = a.rows[3 * r + 2];
+      rl[rb].cs = a.rows[3 * r + 1];
+      rl[rb].ce = a.rows[3 * r + 2];
       orig = a.perm[pos];
     }
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) {
       half8 h;
-      if (valid[rb]) {
+      if (valid) {
         h = *reinterpret_cast<const half8 *>(a.xh + (size_t)orig * DP + kb * 16 + 8 * hi);
         h = h * (_Float16)(-2.0f);
       } else {
@@ -244,107 +318,105 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
       }
       bop[rb][kb] = h;
     }
-    Ni[rb] = valid[rb] ? a.norm[orig] : 0.0f;
-    const float Rs = (valid[rb] ? a.R[orig] : 0.0f) + a.Rmax;
-    Ei[rb] = a.gamma * Rs * Rs * (1.0f + 1e-6f);
-    Di[rb] = (valid[rb] ? a.rho[orig] : 0.0f) + a.rhoMax;
-    runMin[rb] = INFINITY;
-    th[rb] = INFINITY;
-    count[rb] = 0;
+    rl[rb].N = valid ? a.norm[orig] : 0.0f;
+    const float Rs = (valid ? a.R[orig] : 0.0f) + a.Rmax;
+    rl[rb].E = a.gamma * Rs * Rs * (1.0f + 1e-6f);
+    rl[rb].D = (valid ? a.rho[orig] : 0.0f) + a.rhoMax;
+    rl[rb].runMin = INFINITY;
+    rl[rb].rowMin = INFINITY;
+    rl[rb].th = INFINITY;
+    rl[rb].count = 0;
   }
 
-  auto stage = [&](int t, int buf) {
-    const half8 *src = reinterpret_cast<const half8 *>(a.xhp + (size_t)t * kTileCols * DP);
-    half8 *dst = reinterpret_cast<half8 *>(sx[buf]);
-    for (int v = threadIdx.x; v < VEC_PER_TILE; v += 256)
-      dst[v] = src[v];
+  // Column tiles stream through LDS, double-buffered: the global loads of
+  // tile t + 1 are issued before tile t is computed.
+  const half8 *gsrc = reinterpret_cast<const half8 *>(a.xhp);
+  half8 pf[VPT];
+  float pfn = 0.0f;
+  auto fetch = [&](int t) {
+#pragma unroll
+    for (int v = 0; v < VPT; ++v)
+      pf[v] = gsrc[(size_t)t * VEC_PER_TILE + threadIdx.x + v * 256];
     if (threadIdx.x < kTileCols)
-      sn[buf][threadIdx.x] = a.normp[t * kTileCols + threadIdx.x];
+      pfn = a.normp[(size_t)t * kTileCols + threadIdx.x];
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int v = 0; v < VPT; ++v)
+      reinterpret_cast<half8 *>(sx[buf])[threadIdx.x + v * 256] = pf[v];
+    if (threadIdx.x < kTileCols)
+      sn[buf][threadIdx.x] = pfn;
   };
 
   const int numTiles = a.nPad / kTileCols;
-  stage(0, 0);
+  fetch(0);
+  store(0);
   __syncthreads();
   for (int t = 0; t < numTiles; ++t) {
     const int buf = t & 1;
+    if (t + 1 < numTiles)
+      fetch(t + 1);
     const int c0 = t * kTileCols, c1 = c0 + kTileCols;
     // Wave-uniform decisions about the same-component mask.
     bool inter = false, contained = true;
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) {
-      if (!valid[rb])
+      if (rl[rb].slot < 0)
         continue;
-      inter |= ce[rb] > c0 && cs[rb] < c1;
-      contained &= cs[rb] <= c0 && ce[rb] >= c1;
+      inter |= rl[rb].ce > c0 && rl[rb].cs < c1;
+      contained &= rl[rb].cs <= c0 && rl[rb].ce >= c1;
     }
     const bool skipAll = __ballot(!contained) == 0;
     const bool anyMask = __ballot(inter) != 0;
     if (!skipAll) {
-      float tmin[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+      BlockOps<KB> o0, o1;
+      loadBlock<KB>(o0, sx[buf], sn[buf], 0, l32, hi);
 #pragma unroll 1
-      for (int cb = 0; cb < kTileCols / 32; ++cb) {
-        f32x16 seed;
+      for (int cb = 0; cb < kTileCols / 32; cb += 2) {
+        // All four row blocks' MFMA chains are issued before any epilogue
+        // reads a result, so they overlap in the matrix pipeline.
+        f32x16 acc[4];
+        loadBlock<KB>(o1, sx[buf], sn[buf], cb + 1, l32, hi);
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const float4 s4 =
-              *reinterpret_cast<const float4 *>(&sn[buf][cb * 32 + g * 8 + hi * 4]);
-          seed[4 * g + 0] = s4.x;
-          seed[4 * g + 1] = s4.y;
-          seed[4 * g + 2] = s4.z;
-          seed[4 * g + 3] = s4.w;
-        }
-        half8 aop[KB];
+        for (int rb = 0; rb < 4; ++rb)
+          acc[rb] = mfmaBlock<KB>(o0, bop[rb]);
 #pragma unroll
-        for (int kb = 0; kb < KB; ++kb)
-          aop[kb] = *reinterpret_cast<const half8 *>(&sx[buf][(cb * 32 + l32) * DP + kb * 16 +
-                                                               8 * hi]);
+        for (int rb = 0; rb < 4; ++rb)
+          blockEpilogue<KB>(acc[rb], rl[rb], anyMask, c0 + cb * 32, lane, t * 8 + cb * 2 + hi, a,
+                            hi);
+        if (cb + 2 < kTileCols / 32)
+          loadBlock<KB>(o0, sx[buf], sn[buf], cb + 2, l32, hi);
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
-          f32x16 acc = seed;
+        for (int rb = 0; rb < 4; ++rb)
+          acc[rb] = mfmaBlock<KB>(o1, bop[rb]);
 #pragma unroll
-          for (int kb = 0; kb < KB; ++kb)
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(aop[kb], bop[rb][kb], acc, 0, 0, 0);
-          if (anyMask) {
-            const unsigned span = (unsigned)(ce[rb] - cs[rb]);
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-              const int j = c0 + cb * 32 + mOfReg(r, lane);
-              if ((unsigned)(j - cs[rb]) < span)
-                acc[r] = INFINITY;
-            }
-          }
-          float m = tmin[rb];
-#pragma unroll
-          for (int r = 0; r < 16; r += 2)
-            m = fminf(m, fminf(acc[r], acc[r + 1]));
-          tmin[rb] = m;
-        }
+        for (int rb = 0; rb < 4; ++rb)
+          blockEpilogue<KB>(acc[rb], rl[rb], anyMask, c0 + (cb + 1) * 32, lane,
+                            t * 8 + (cb + 1) * 2 + hi, a, hi);
       }
+      // Tile end: both halves of each row agree on the row minimum and theta.
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) {
-        const float tm = fminf(tmin[rb], __shfl_xor(tmin[rb], 32, 64));
-        if (tm < runMin[rb]) {
-          runMin[rb] = tm;
-          th[rb] = thetaOf(tm, Ni[rb], Ei[rb], Di[rb], sqrtKappa);
-        }
-        if (hi == 0 && valid[rb] && tm < INFINITY && tm <= th[rb]) {
-          if (count[rb] < kCandCap)
-            a.cand[(size_t)rowSlot[rb] * kCandCap + count[rb]] = make_uint2(t, __float_as_uint(tm));
-          ++count[rb];
+        const float m = fminf(rl[rb].runMin, __shfl_xor(rl[rb].runMin, 32, 64));
+        rl[rb].runMin = m;
+        if (m < rl[rb].rowMin) {
+          rl[rb].rowMin = m;
+          rl[rb].th = thetaOf(m, rl[rb].N, rl[rb].E, rl[rb].D, sqrtKappa);
         }
       }
     }
     if (t + 1 < numTiles)
-      stage(t + 1, buf ^ 1);
+      store(buf ^ 1);
     __syncthreads();
   }
   unsigned long long rec = 0;
 #pragma unroll
   for (int rb = 0; rb < 4; ++rb)
-    if (hi == 0 && valid[rb]) {
-      a.candCount[rowSlot[rb]] = count[rb];
-      a.theta[rowSlot[rb]] = th[rb];
-      rec += count[rb];
+    if (rl[rb].slot >= 0) {
+      a.candCount[(size_t)rl[rb].slot * 2 + hi] = rl[rb].count;
+      if (hi == 0)
+        a.theta[rl[rb].slot] = rl[rb].th;
+      rec += rl[rb].count;
     }
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1)
@@ -354,7 +426,7 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Exact FP32 recheck: one wave per row.
+// Exact FP32 recheck: one wave per row; 16 lanes per recorded half-block.
 
 struct RecheckArgs {
   const float *X;
@@ -367,9 +439,10 @@ struct RecheckArgs {
   const float *theta;
   bool sqeuclidean;
   unsigned long long *nearest; // by original index
-  unsigned long long *stats;   // [1] tiles rechecked, [2] pairs, [3] overflow rows
+  unsigned long long *stats;   // [1] half-blocks rechecked, [2] pairs, [3] overflow rows
 };
 
+// FP32 direct-form weight of (row in LDS, column position col): the plain fold.
 DEV unsigned long long recheckColumn(const RecheckArgs &a, const float *xi, int col, int cs,
                                      int ce) {
   if (col >= a.n || (col >= cs && col < ce))
@@ -377,51 +450,87 @@ DEV unsigned long long recheckColumn(const RecheckArgs &a, const float *xi, int 
   const int j = a.perm[col];
   const float *xj = a.X + (size_t)j * a.d;
   float acc = 0.0f;
-  for (int k = 0; k < a.d; ++k) {
-    const float diff = xi[k] - xj[k];
-    acc = fmaf(diff, diff, acc);
+  if ((a.d & 3) == 0) {
+    for (int k = 0; k < a.d; k += 4) {
+      const float4 v = *reinterpret_cast<const float4 *>(xj + k);
+      float diff = xi[k] - v.x;
+      acc = fmaf(diff, diff, acc);
+      diff = xi[k + 1] - v.y;
+      acc = fmaf(diff, diff, acc);
+      diff = xi[k + 2] - v.z;
+      acc = fmaf(diff, diff, acc);
+      diff = xi[k + 3] - v.w;
+      acc = fmaf(diff, diff, acc);
+    }
+  } else {
+    for (int k = 0; k < a.d; ++k) {
+      const float diff = xi[k] - xj[k];
+      acc = fmaf(diff, diff, acc);
+    }
   }
   const float w = a.sqeuclidean ? acc : sqrtf(acc);
   return w != INFINITY ? packKey(w, (unsigned)j) : kEmptyKey;
 }
 
 __global__ void __launch_bounds__(256) recheckKernel(RecheckArgs a) {
-  const int lane = threadIdx.x & 63;
-  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  __shared__ float rowX[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = blockIdx.x * 4 + wave;
   if (r >= a.numRows)
-    return; // wave-uniform
+    return; // wave-uniform; no block-wide barrier below
   const int pos = a.rows[3 * r], cs = a.rows[3 * r + 1], ce = a.rows[3 * r + 2];
   const int orig = a.perm[pos];
-  const float *xi = a.X + (size_t)orig * a.d;
-  const int count = a.candCount[r];
+  float *xi = rowX[wave];
+  for (int k = lane; k < a.d; k += 64)
+    xi[k] = a.X[(size_t)orig * a.d + k];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int c0 = a.candCount[(size_t)r * 2], c1 = a.candCount[(size_t)r * 2 + 1];
   const float th = a.theta[r];
   unsigned long long best = kEmptyKey;
-  unsigned long long tiles = 0;
-  if (count > kCandCap) {
+  unsigned long long blocks = 0;
+  const bool overflow = c0 > kCandCap || c1 > kCandCap;
+  if (overflow) {
     for (int col = lane; col < a.n; col += 64)
       best = minKey(best, recheckColumn(a, xi, col, cs, ce));
-    tiles = (a.n + kTileCols - 1) / kTileCols;
   } else {
-    for (int e = 0; e < count; ++e) {
-      const uint2 c = a.cand[(size_t)r * kCandCap + e];
-      if (!(__uint_as_float(c.y) <= th))
-        continue;
-      ++tiles;
-      const int c0 = (int)c.x * kTileCols;
-      best = minKey(best, recheckColumn(a, xi, c0 + lane, cs, ce));
-      best = minKey(best, recheckColumn(a, xi, c0 + 64 + lane, cs, ce));
+    // Entry e: list e / ..., 16 lanes each; half-block b = tile * 8 + cb * 2 + hi
+    // covers columns tile*128 + cb*32 + g*8 + hi*4 + q, g, q in 0..3.
+    const int total = c0 + c1;
+    const int sub = lane >> 4, idx = lane & 15;
+    for (int e0 = 0; e0 < total; e0 += 4) {
+      const int e = e0 + sub;
+      unsigned long long k = kEmptyKey;
+      if (e < total) {
+        const uint2 c = e < c0 ? a.cand[((size_t)r * 2) * kCandCap + e]
+                               : a.cand[((size_t)r * 2 + 1) * kCandCap + (e - c0)];
+        if (__uint_as_float(c.y) <= th) {
+          const int blk = (int)c.x;
+          const int col = (blk >> 3) * kTileCols + ((blk >> 1) & 3) * 32 + (idx >> 2) * 8 +
+                          (blk & 1) * 4 + (idx & 3);
+          k = recheckColumn(a, xi, col, cs, ce);
+          if (idx == 0)
+            ++blocks;
+        }
+      }
+      best = minKey(best, k);
     }
   }
 #pragma unroll
-  for (int m = 32; m > 0; m >>= 1)
+  for (int m = 32; m > 0; m >>= 1) {
     best = minKey(best, shflXorKey(best, m));
+    blocks += shflXorKey(blocks, m);
+  }
   if (lane == 0) {
     a.nearest[orig] = best;
-    if (tiles)
-      atomicAdd(&a.stats[1], tiles);
-    atomicAdd(&a.stats[2], tiles * kTileCols);
-    if (count > kCandCap)
+    if (overflow) {
       atomicAdd(&a.stats[3], 1ull);
+      atomicAdd(&a.stats[2], (unsigned long long)a.n);
+    } else if (blocks) {
+      atomicAdd(&a.stats[1], blocks);
+      atomicAdd(&a.stats[2], blocks * 16);
+    }
   }
 }
 
@@ -485,9 +594,9 @@ MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
   s(mp->rho.alloc(n));
   s(mp->perm.alloc(n));
   s(mp->rows.alloc((size_t)3 * n));
-  s(mp->candCount.alloc(n));
+  s(mp->candCount.alloc((size_t)2 * n));
   s(mp->theta.alloc(n));
-  s(mp->cand.alloc((size_t)n * kCandCap));
+  s(mp->cand.alloc((size_t)2 * n * kCandCap));
   s(mp->stats.alloc(4));
   if (s.ok()) {
     s(fill(sums, 0, d * sizeof(double)));
