@@ -65,8 +65,86 @@ def cpu_baseline(d, seed, sample_n, full_n):
                       f"(d={d}) in {t:.2f} s, extrapolated to n={full_n} by n^2"}
 
 
+def make_c3(seed=2):
+    """20 isotropic Gaussian blobs in 8-d (centres uniform in [-10, 10], std 1,
+    45,000 points each) plus 100,000 uniform noise points over their bounding box."""
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(-10, 10, (20, 8))
+    blobs = [c + rng.standard_normal((45_000, 8)) for c in centres]
+    X = np.vstack(blobs)
+    noise = rng.uniform(X.min(0), X.max(0), (100_000, 8))
+    return np.vstack([X, noise]).astype(np.float32)
+
+
+def other_config(name, args):
+    """Seconds per call for the other configurations (parity-test cases of the
+    same path; printed as one JSON line, not the headline)."""
+    from cuhc_amd import _lib
+    from cuhc_amd._hdbscan import tree_to_labels
+    lib = _lib.lib
+    out = {"config": name}
+    if name == "c3":
+        X = make_c3()
+        n, d, k = len(X), 8, 25
+        dev = _lib.DeviceArray(X)
+        tree = np.empty((n - 1, 4))
+        core = np.empty(n, np.float32)
+        methods = {"auto": 0, "fused": 1, "radix": 2, "learned": 3}
+        trees = {}
+        for method in args.core_methods.split(","):
+            call = lambda: _lib.check(lib.cuhc_hdbscan_tree_device(  # noqa: E731
+                dev.ptr, n, d, 0, 2.0, k, 1.0, methods[method], _lib.double_ptr(tree),
+                _lib.float_ptr(core)))
+            call()
+            _lib.set_timing(True)
+            _lib.reset_timings()
+            _lib.prune_stats()
+            _lib.learned_stats()
+            t = timed_steps(call, args.steps)
+            out[method] = {"tree_s": t, "phases_ms": {
+                k2: round(v[0] / args.steps, 2) for k2, v in _lib.read_timings().items() if v[1]},
+                "prune_stats": _lib.prune_stats()}
+            if method == "learned":
+                out[method]["learned_stats_per_call"] = {
+                    k2: v / args.steps for k2, v in _lib.learned_stats().items()}
+            _lib.set_timing(False)
+            trees[method] = (tree.copy(), core.copy())
+            log(f"c3 {method}: {out[method]}")
+        first = next(iter(trees.values()))
+        out["identical_across_methods"] = all(
+            np.array_equal(a, first[0]) and np.array_equal(c, first[1])
+            for a, c in trees.values())
+        t0 = time.perf_counter()
+        labels, _ = tree_to_labels(tree, 25)
+        out.update(extraction_s=time.perf_counter() - t0,
+                   clusters=int(labels.max() + 1), noise=int((labels < 0).sum()),
+                   workload="HDBSCAN mcs=min_samples=25, n=1,000,000, d=8, 20 blobs + 10% noise")
+    else:
+        spec = {"c1": (20_000, 32, 0, 0, 1, 0, "single/mst, n=20k, d=32"),
+                "c2": (100_000, 32, 1, 3, 3, 0, "ward/rnn, n=100k, d=32"),
+                "c5": (100_000, 64, 4, 2, 0, 5, "average/dense cosine, n=100k, d=64")}[name]
+        n, d, seed, linkage_id, algorithm, metric, label = spec
+        X = make_points(n, d, seed)
+        if name == "c5":
+            X /= np.linalg.norm(X, axis=1, keepdims=True)
+        dev = _lib.DeviceArray(X)
+        tree = np.empty((n - 1, 4))
+        call = lambda: _lib.check(lib.cuhc_linkage_device(  # noqa: E731
+            dev.ptr, n, d, metric, 2.0, linkage_id, algorithm, _lib.double_ptr(tree)))
+        call()
+        _lib.set_timing(True)
+        _lib.reset_timings()
+        t = timed_steps(call, args.steps)
+        out["phases_ms"] = {k2: round(v[0] / args.steps, 2) for k2, v in
+                            _lib.read_timings().items() if v[1]}
+        _lib.set_timing(False)
+        out.update(tree_s=t, workload=label)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     parser = argparse.ArgumentParser(description=__doc__)
+    parser.add_argument("--config", default="c4", choices=["c1", "c2", "c3", "c4", "c5"])
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=3)
     parser.add_argument("--warmup", type=int, default=1)
@@ -76,6 +154,8 @@ def main():
     parser.add_argument("--plain-steps", type=int, default=1)
     parser.add_argument("--cpu-sample", type=int, default=20_000)
     parser.add_argument("--no-cpu", action="store_true")
+    parser.add_argument("--core-methods", default="auto,learned",
+                        help="c3: comma-separated core-distance methods to time")
     args = parser.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); "
@@ -85,6 +165,9 @@ def main():
     import cuhc_amd  # noqa: F401
     from cuhc_amd import _lib
 
+    if args.config != "c4":
+        other_config(args.config, args)
+        return
     n, d = args.n, args.d
     log(f"generating n={n} d={d} seed={args.seed}")
     X = make_points(n, d, args.seed)

@@ -17,7 +17,7 @@ from . import _lib
 from ._distances import METRIC_IDS, as_points, check_distance_range, validate_metric
 
 NOISE = -1
-CORE_METHOD_IDS = {"auto": 0, "fused": 1, "radix": 2}
+CORE_METHOD_IDS = {"auto": 0, "fused": 1, "radix": 2, "learned": 3}
 
 
 def hdbscan_tree(X, min_samples, metric="euclidean", alpha=1.0, p=2.0, core_method="auto"):

@@ -57,6 +57,12 @@ _declare("cuhc_set_timing", _int, _int)
 _declare("cuhc_read_timings", None, _f64p, _i64p, _i64p, _int)
 _declare("cuhc_reset_timings", None)
 _declare("cuhc_mixed_stats", None, _u64p, _u64p, _u64p, _u64p)
+_declare("cuhc_learned_stats", None, _u64p, _u64p, _u64p, _u64p, _u64p)
+_declare("cuhc_set_learned_training", _int, _int)
+_declare("cuhc_mlp_param_count", _int)
+_declare("cuhc_mlp_gradient", _int, _f32p, _f32p, _f32p, _int, _f32p, _f32p)
+_declare("cuhc_mlp_forward", _int, _f32p, _f32p, _int, _f32p)
+_declare("cuhc_mlp_train", _int, _f32p, _f32p, _f32p, _int, _int, _float)
 
 PRUNE_MODES = {"auto": 0, "always": 1, "never": 2}
 PRECISION_MODES = {"auto": 0, "fp32": 1, "mixed": 2}
@@ -95,6 +101,14 @@ def mixed_stats():
     vals = [ctypes.c_ulonglong() for _ in range(4)]
     lib.cuhc_mixed_stats(*[ctypes.byref(v) for v in vals])
     return dict(zip(["recorded", "rechecked_tiles", "rechecked_pairs", "overflow_rows"],
+                    [v.value for v in vals]))
+
+
+def learned_stats():
+    """Counters of the learned-radius core distances since the last call (then reset)."""
+    vals = [ctypes.c_ulonglong() for _ in range(5)]
+    lib.cuhc_learned_stats(*[ctypes.byref(v) for v in vals])
+    return dict(zip(["proposals", "hits", "fallbacks", "candidates", "tiles"],
                     [v.value for v in vals]))
 
 
@@ -148,3 +162,52 @@ def double_ptr(array):
 
 def int64_ptr(array):
     return array.ctypes.data_as(_i64p)
+
+
+def set_learned_training(steps):
+    """Adam steps that train the radius network per learned-core call (0 = untrained)."""
+    check(lib.cuhc_set_learned_training(int(steps)))
+
+
+MLP_PARAMS = lib.cuhc_mlp_param_count()
+
+
+def _mlp_inputs(params, X):
+    import numpy as np
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    if params.shape != (MLP_PARAMS,) or X.ndim != 2 or X.shape[1] != 4:
+        raise ValueError(f"params must have {MLP_PARAMS} entries and X shape (m, 4)")
+    return params, X
+
+
+def mlp_forward(params, X):
+    """Outputs of the radius network for the rows of X (m x 4), computed on the GPU."""
+    import numpy as np
+    params, X = _mlp_inputs(params, X)
+    y = np.empty(len(X), dtype=np.float32)
+    check(lib.cuhc_mlp_forward(float_ptr(params), float_ptr(X), len(X), float_ptr(y)))
+    return y
+
+
+def mlp_gradient(params, X, t):
+    """(gradient, loss) of the mean 0.5 (y - t)^2, from the GPU backward kernels."""
+    import numpy as np
+    params, X = _mlp_inputs(params, X)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    grad = np.empty(MLP_PARAMS, dtype=np.float32)
+    loss = ctypes.c_float()
+    check(lib.cuhc_mlp_gradient(float_ptr(params), float_ptr(X), float_ptr(t), len(X),
+                                float_ptr(grad), ctypes.byref(loss)))
+    return grad, loss.value
+
+
+def mlp_train(params, X, t, steps, lr=0.01):
+    """Parameters after `steps` full-batch Adam updates on the GPU (input unchanged)."""
+    params, X = _mlp_inputs(params, X)
+    params = params.copy()
+    import numpy as np
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    check(lib.cuhc_mlp_train(float_ptr(params), float_ptr(X), float_ptr(t), len(X), int(steps),
+                             float(lr)))
+    return params

@@ -154,7 +154,7 @@ int singleLinkageMst(const DevicePoints &pts, int n, int d, MetricSpec m, bool p
   std::vector<Edge> edges;
   int status = boruvkaMst(pts.data(), n, d, m, WeightSpec{}, opt, edges);
   if (status == 0)
-    singleLinkageFromMst(std::move(edges), n, tree);
+    status = treeFromMst(std::move(edges), n, tree);
   return status;
 }
 
@@ -185,7 +185,7 @@ int hdbscanTree(const DevicePoints &pts, int n, int d, MetricSpec m, int minSamp
       hostCore[pts.ordered() ? pts.order[i] : i] = c[i];
   }
   if (s.ok())
-    singleLinkageFromMst(std::move(edges), n, tree);
+    s(treeFromMst(std::move(edges), n, tree));
   return s.code;
 }
 

@@ -92,15 +92,19 @@ int hdbscanImpl(const float *X, bool onDevice, int n, int d, int metricId, float
                 int minSamples, float alpha, int coreMethod, double *tree, float *core) {
   if (n <= 0 || d <= 0 || minSamples < 1 || minSamples > n)
     return kErrInvalidShape;
-  if (coreMethod < 0 || coreMethod > 2 || (coreMethod == kCoreFused && minSamples > kMaxFusedK))
+  if (coreMethod < 0 || coreMethod > 3 || (coreMethod == kCoreFused && minSamples > kMaxFusedK))
     return kErrInvalidAlgorithm;
   if (!validMetric(metricId))
     return kErrInvalidMetric;
   const MetricSpec m{metricId, metricParam};
   bool prune = shouldPrune(gPruneMode.load(), metricId, n, d);
+  // The learned radii are predicted from k-d tiles: spatial order pays for it
+  // at any size (the results are the same in every order).
+  const bool spatial = prune || (coreMethod == kCoreLearned && n > 1 &&
+                                 gPruneMode.load() != kPruneNever && metricHasBoxBound(metricId));
   DevicePoints pts;
   Status s;
-  s(onDevice ? pts.adopt(X, n, d, prune) : pts.upload(X, n, d, prune));
+  s(onDevice ? pts.adopt(X, n, d, spatial) : pts.upload(X, n, d, spatial));
   if (s.ok())
     s(decidePruning(pts, n, d, m, prune));
   PassStats stats;

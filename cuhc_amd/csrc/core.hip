@@ -60,7 +60,7 @@ DEV void insertSorted(float *list, int k, float v) {
 template <class T, class Metric>
 __global__ void __launch_bounds__(256)
     kthNearestFusedKernel(const float *__restrict__ X, int n, int d, Metric metric, int k,
-                          float *__restrict__ core, CoreBoxes boxes) {
+                          float *__restrict__ core, CoreBoxes boxes, int stepLimit) {
   constexpr int TR = T::TR;
   __shared__ __align__(16) TileSmem<T> sm;
   __shared__ int skip;
@@ -82,7 +82,8 @@ __global__ void __launch_bounds__(256)
   const int lane = threadIdx.x & 63;
   const unsigned long long groupMask = 0xffffull << (lane & 48);
 
-  for (int step = 0; step < 2 * numTiles; ++step) {
+  const int steps = min(2 * numTiles, stepLimit);
+  for (int step = 0; step < steps; ++step) {
     const int colTile = walkTileCore(blockIdx.x, step, numTiles);
     if (colTile < 0)
       continue;
@@ -172,13 +173,16 @@ __global__ void coreBoxesKernel(const float *__restrict__ X, int n, int d, int t
   }
 }
 
-// One workgroup (256 threads) per row of the distance block.
+// One workgroup (256 threads) per row of the distance block. The block's row r
+// is point rowList[firstRow + r] (or firstRow + r without a list); its core
+// distance goes to core[that point].
 __global__ void __launch_bounds__(256)
     kthSmallestRowKernel(const float *__restrict__ dist, int numCols, int firstRow, int k,
-                         float *__restrict__ core) {
+                         const int *__restrict__ rowList, float *__restrict__ core) {
   __shared__ unsigned int hist[4][256];
   __shared__ unsigned int prefix, prefixMask, remaining;
-  const int row = blockIdx.x, self = firstRow + row;
+  const int row = blockIdx.x;
+  const int self = rowList ? rowList[firstRow + row] : firstRow + row;
   const float *rd = dist + (size_t)row * numCols;
   const int wave = threadIdx.x >> 6;
   if (threadIdx.x == 0) {
@@ -229,7 +233,8 @@ template <class F> int withMetricCore(MetricSpec m, F &&f) {
   }
 }
 
-int coreFused(const float *X, int n, int d, MetricSpec m, int k, float *core, bool prune) {
+int coreFused(const float *X, int n, int d, MetricSpec m, int k, float *core, bool prune,
+              int stepLimit = 0x7fffffff) {
   return withMetricCore(m, [&](auto metric) {
     using Metric = decltype(metric);
     auto run = [&](auto fill) {
@@ -251,7 +256,7 @@ int coreFused(const float *X, int n, int d, MetricSpec m, int k, float *core, bo
       }
       PhaseTimer timer(kPhaseCore);
       hipLaunchKernelGGL((kthNearestFusedKernel<T, Metric>), dim3(tiles), dim3(256), smem,
-                         stream(), X, n, d, metric, k, core, boxes);
+                         stream(), X, n, d, metric, k, core, boxes, stepLimit);
       s((int)hipGetLastError());
       s(sync());
       return s.code;
@@ -277,7 +282,7 @@ int coreRadix(const float *X, int n, int d, MetricSpec m, int k, float *core) {
     const int count = std::min(rows, n - first);
     s(pairwiseMatrix(X + (size_t)first * d, count, X, n, d, m, chunk));
     hipLaunchKernelGGL(kthSmallestRowKernel, dim3(count), dim3(256), 0, stream(), chunk.get(), n,
-                       first, k, core);
+                       first, k, nullptr, core);
     s((int)hipGetLastError());
   }
   s(sync());
@@ -286,12 +291,41 @@ int coreRadix(const float *X, int n, int d, MetricSpec m, int k, float *core) {
 
 } // namespace
 
+int coreLocal(const float *X, int n, int d, MetricSpec m, int k, int stepLimit, float *out) {
+  return coreFused(X, n, d, m, std::min(k, kMaxFusedK), out, false, stepLimit);
+}
+
+int coreForRows(const float *X, int n, int d, MetricSpec m, int k, const int *rows, int numRows,
+                float *core) {
+  if (numRows <= 0)
+    return 0;
+  const size_t maxBuffer = (size_t)512 << 20;
+  const size_t rowBytes = (size_t)n * sizeof(float);
+  int chunkRows = (int)std::max<size_t>(1, std::min<size_t>(maxBuffer / rowBytes, numRows));
+  DevArray<float> chunk, rowData;
+  Status s;
+  s(chunk.alloc((size_t)chunkRows * n));
+  s(rowData.alloc((size_t)chunkRows * d));
+  for (int first = 0; s.ok() && first < numRows; first += chunkRows) {
+    const int count = std::min(chunkRows, numRows - first);
+    s(gatherRows(X, rows + first, count, d, rowData));
+    s(pairwiseMatrix(rowData, count, X, n, d, m, chunk));
+    hipLaunchKernelGGL(kthSmallestRowKernel, dim3(count), dim3(256), 0, stream(), chunk.get(), n,
+                       first, k, rows, core);
+    s((int)hipGetLastError());
+  }
+  s(sync());
+  return s.code;
+}
+
 int coreDistances(const float *X, int n, int d, MetricSpec m, int k, float *core, int method,
                   bool prune) {
   if (k < 1 || k > n)
     return kErrInvalidShape;
   if (method == kCoreFused && k > kMaxFusedK)
     return kErrInvalidAlgorithm;
+  if (method == kCoreLearned)
+    return coreLearned(X, n, d, m, k, core, prune);
   if (method == kCoreFused || (method == kCoreAuto && k <= kMaxFusedK))
     return coreFused(X, n, d, m, k, core, prune);
   return coreRadix(X, n, d, m, k, core);

@@ -34,28 +34,40 @@ bool mergesFormTree(const std::vector<Edge> &merges, int n) {
 }
 
 void treeFromOrderedMerges(const std::vector<Edge> &merges, int n, double *tree) {
-  UnionFind sets(n);
-  std::vector<int> id(n), size(n, 1); // indexed by set root
-  std::iota(id.begin(), id.end(), 0);
+  // One record per point keeps parent, set size and cluster id in one cache
+  // line access (the finds are random over n points).
+  struct Node {
+    int parent, size, id;
+  };
+  std::vector<Node> node(n);
+  for (int i = 0; i < n; ++i)
+    node[i] = {i, 1, i};
+  auto find = [&](int x) {
+    while (node[x].parent != x) {
+      const int up = node[node[x].parent].parent;
+      node[x].parent = up;
+      x = up;
+    }
+    return x;
+  };
   for (size_t s = 0; s < merges.size(); ++s) {
-    const int ru = sets.find(merges[s].u), rv = sets.find(merges[s].v);
-    const int a = id[ru], b = id[rv];
-    const int total = size[ru] + size[rv];
+    int ru = find(merges[s].u), rv = find(merges[s].v);
+    const int a = node[ru].id, b = node[rv].id;
+    const int total = node[ru].size + node[rv].size;
     double *row = tree + s * 4;
     row[0] = std::min(a, b);
     row[1] = std::max(a, b);
     row[2] = merges[s].w;
     row[3] = total;
-    const int root = sets.unite(ru, rv);
-    id[root] = n + (int)s;
-    size[root] = total;
+    if (node[ru].size < node[rv].size)
+      std::swap(ru, rv);
+    node[rv].parent = ru;
+    node[ru].size = total;
+    node[ru].id = n + (int)s;
   }
 }
 
-void singleLinkageFromMst(std::vector<Edge> edges, int n, double *tree) {
-  std::sort(edges.begin(), edges.end(), edgeLess);
-  treeFromOrderedMerges(edges, n, tree);
-}
+void sortEdges(std::vector<Edge> &edges) { std::sort(edges.begin(), edges.end(), edgeLess); }
 
 void orderChainMerges(std::vector<Edge> &merges, int n) {
   UnionFind sets(n);

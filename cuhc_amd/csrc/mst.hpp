@@ -38,10 +38,25 @@ int boruvkaMst(const float *X, int n, int d, MetricSpec m, WeightSpec w, const M
                std::vector<Edge> &edges);
 
 // Core distances: core[i] = k-th smallest distance from point i (itself = 1st).
-enum CoreMethod : int { kCoreAuto = 0, kCoreFused = 1, kCoreRadix = 2 };
+// kCoreLearned (an extension): radius proposals from the trained network seed
+// the pruned gather and a radix select; rows whose proposal fails fall back to
+// the exact full search. All methods give identical values.
+enum CoreMethod : int { kCoreAuto = 0, kCoreFused = 1, kCoreRadix = 2, kCoreLearned = 3 };
 constexpr int kMaxFusedK = 64;
 int coreDistances(const float *X, int n, int d, MetricSpec m, int k, float *core, int method,
                   bool prune);
+// The k-th smallest over the first stepLimit walk steps only (own tile, then
+// neighbours): a local density feature.
+int coreLocal(const float *X, int n, int d, MetricSpec m, int k, int stepLimit, float *out);
+// Exact core distances of the listed points (device positions) against all points.
+int coreForRows(const float *X, int n, int d, MetricSpec m, int k, const int *rows, int numRows,
+                float *core);
+int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, bool spatial);
+
+struct LearnedStats {
+  unsigned long long proposals = 0, hits = 0, fallbacks = 0, candidates = 0, tiles = 0;
+};
+void learnedStatsAdd(const LearnedStats &s);
 
 // The single-linkage tree of mutual reachability (host tree, host core in
 // original order).

@@ -69,9 +69,15 @@ void setTiming(bool on) {
   gTiming = on;
 }
 
-PhaseTimer::PhaseTimer(int p) : phase(p), active(gTiming) {
+thread_local int tDepth[kNumPhases];
+
+PhaseTimer::PhaseTimer(int p) : phase(p), active(gTiming), counted(gTiming) {
   if (!active)
     return;
+  if (tDepth[p]++ > 0) {
+    active = false;
+    return;
+  }
   PendingEvent e{p, nullptr, nullptr};
   if (hipEventCreate(&e.start) != hipSuccess || hipEventCreate(&e.stop) != hipSuccess) {
     active = false;
@@ -83,6 +89,8 @@ PhaseTimer::PhaseTimer(int p) : phase(p), active(gTiming) {
 }
 
 PhaseTimer::~PhaseTimer() {
+  if (counted)
+    --tDepth[phase];
   if (!active)
     return;
   std::lock_guard<std::mutex> lock(gTimingMutex);

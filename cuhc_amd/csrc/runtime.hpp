@@ -73,9 +73,12 @@ enum Phase : int {
   kNumPhases = 8,
 };
 
+// Times the enclosed launches of one phase; a timer nested inside another of
+// the same phase (on the same host thread) records nothing.
 struct PhaseTimer {
   int phase;
   bool active;
+  bool counted;
   explicit PhaseTimer(int phase);
   ~PhaseTimer();
 };

@@ -15,7 +15,8 @@
  * 4 minkowski (metricParam = p), 5 cosine. linkage: 0 single, 1 complete,
  * 2 average, 3 ward. algorithm: 0 dense, 1 mst (single only), 2 nn_chain and
  * 3 rnn (ward + euclidean only). core method: 0 auto, 1 fused (k <= 64),
- * 2 radix. prune mode: 0 auto, 1 always, 2 never.
+ * 2 radix, 3 learned radius (an extension; same values). prune mode: 0 auto,
+ * 1 always, 2 never.
  */
 #ifndef CUHC_H
 #define CUHC_H
@@ -96,6 +97,29 @@ int cuhc_set_timing(int on);
 /* Fills up to count phases: total ms, launches, algorithmic units (pairs). */
 void cuhc_read_timings(double *ms, long long *launches, long long *units, int count);
 void cuhc_reset_timings(void);
+
+/* Learned-radius core distances (cuhc_hdbscan_tree coreMethod 3): counters
+ * since the last call (then resets): radius proposals, proposals that held
+ * (k neighbours inside the radius, list not overflowing), exact fallbacks, and
+ * in-radius candidates gathered, distance tiles computed by the gather. */
+void cuhc_learned_stats(unsigned long long *proposals, unsigned long long *hits,
+                        unsigned long long *fallbacks, unsigned long long *candidates,
+                        unsigned long long *tiles);
+/* Adam steps used to train the radius network per call (default 300; 0 keeps
+ * the untrained initialisation — a test hook that measures what training
+ * gains). Returns -2 for negative values. */
+int cuhc_set_learned_training(int steps);
+
+/* The radius-proposal network (4 inputs, two tanh layers of 32, one output),
+ * host arrays in and out, for tests and offline use. Parameters:
+ * cuhc_mlp_param_count() floats, layout W1[32][4], b1, W2[32][32], b2, W3[32], b3.
+ * Gradient of the mean of 0.5 (y - t)^2 over m samples; loss may be NULL. */
+int cuhc_mlp_param_count(void);
+int cuhc_mlp_gradient(const float *params, const float *X, const float *t, int m, float *grad,
+                      float *loss);
+int cuhc_mlp_forward(const float *params, const float *X, int m, float *y);
+/* Full-batch Adam, `steps` updates of params (in place). */
+int cuhc_mlp_train(float *params, const float *X, const float *t, int m, int steps, float lr);
 
 /* Counters of the mixed-precision pass since the last call (then resets):
  * candidate tiles recorded, tiles rechecked in FP32, pairs rechecked,

@@ -35,6 +35,8 @@ def lib():
                                     f32p],
             "oracle_dense_on_matrix": [f32p, c_int, c_int, f64p],
             "oracle_dense_linkage": [f32p, c_int, c_int, c_int, c_float, c_int, f64p],
+            "oracle_mlp_forward": [f32p, f32p, c_int, f64p],
+            "oracle_mlp_gradient": [f32p, f32p, f32p, c_int, f64p, f64p],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)
@@ -121,3 +123,29 @@ def dense_on_matrix(D, method):
     _check(lib().oracle_dense_on_matrix(dp, len(D), LINKAGES[method],
                                         tree.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
     return tree
+
+
+MLP_PARAMS = 32 * 4 + 32 + 32 * 32 + 32 + 32 + 1
+
+
+def mlp_forward(params, X):
+    """FP64 outputs of the radius network (layout of cuhc_mlp_*)."""
+    params, pp = _f32(params)
+    X, xp = _f32(X)
+    y = np.empty(len(X), dtype=np.float64)
+    _check(lib().oracle_mlp_forward(pp, xp, len(X),
+                                    y.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return y
+
+
+def mlp_gradient(params, X, t):
+    """(FP64 gradient of the mean 0.5 (y - t)^2, loss)."""
+    params, pp = _f32(params)
+    X, xp = _f32(X)
+    t, tp = _f32(t)
+    g = np.empty(MLP_PARAMS, dtype=np.float64)
+    loss = ctypes.c_double()
+    _check(lib().oracle_mlp_gradient(pp, xp, tp, len(X),
+                                     g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                     ctypes.byref(loss)))
+    return g, loss.value

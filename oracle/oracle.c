@@ -415,6 +415,82 @@ int oracle_dense_on_matrix(float *D, int n, int linkage, double *tree) {
   return 0;
 }
 
+/* ---- radius-proposal network ------------------------------------------------
+ * 4 inputs -> 32 tanh -> 32 tanh -> 1; parameters W1[32][4], b1, W2[32][32],
+ * b2, W3[32], b3 (the layout of cuhc_mlp_*). Loss: mean of 0.5 (y - t)^2.
+ * Forward and backward in FP64 by the chain rule, written out directly. */
+#define MLP_F 4
+#define MLP_H 32
+
+static void mlp_forward1(const float *p, const float *x, double *h1, double *h2, double *y) {
+  const float *W1 = p, *b1 = W1 + MLP_H * MLP_F, *W2 = b1 + MLP_H, *b2 = W2 + MLP_H * MLP_H;
+  const float *W3 = b2 + MLP_H, *b3 = W3 + MLP_H;
+  for (int h = 0; h < MLP_H; ++h) {
+    double z = b1[h];
+    for (int f = 0; f < MLP_F; ++f)
+      z += (double)W1[h * MLP_F + f] * x[f];
+    h1[h] = tanh(z);
+  }
+  for (int h = 0; h < MLP_H; ++h) {
+    double z = b2[h];
+    for (int g = 0; g < MLP_H; ++g)
+      z += (double)W2[h * MLP_H + g] * h1[g];
+    h2[h] = tanh(z);
+  }
+  double out = b3[0];
+  for (int h = 0; h < MLP_H; ++h)
+    out += (double)W3[h] * h2[h];
+  *y = out;
+}
+
+int oracle_mlp_forward(const float *p, const float *X, int m, double *y) {
+  double h1[MLP_H], h2[MLP_H];
+  for (int s = 0; s < m; ++s)
+    mlp_forward1(p, X + (size_t)s * MLP_F, h1, h2, &y[s]);
+  return 0;
+}
+
+int oracle_mlp_gradient(const float *p, const float *X, const float *t, int m, double *grad,
+                        double *loss) {
+  const int P = MLP_H * MLP_F + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H + 1;
+  const float *W2 = p + MLP_H * MLP_F + MLP_H, *W3 = W2 + MLP_H * MLP_H + MLP_H;
+  double *gW1 = grad, *gb1 = gW1 + MLP_H * MLP_F, *gW2 = gb1 + MLP_H, *gb2 = gW2 + MLP_H * MLP_H;
+  double *gW3 = gb2 + MLP_H, *gb3 = gW3 + MLP_H;
+  for (int q = 0; q < P; ++q)
+    grad[q] = 0.0;
+  double total = 0.0;
+  for (int s = 0; s < m; ++s) {
+    const float *x = X + (size_t)s * MLP_F;
+    double h1[MLP_H], h2[MLP_H], y, dz2[MLP_H], dz1[MLP_H];
+    mlp_forward1(p, x, h1, h2, &y);
+    const double dy = y - t[s];
+    total += 0.5 * dy * dy;
+    for (int h = 0; h < MLP_H; ++h)
+      dz2[h] = dy * W3[h] * (1.0 - h2[h] * h2[h]);
+    for (int g = 0; g < MLP_H; ++g) {
+      double back = 0.0;
+      for (int h = 0; h < MLP_H; ++h)
+        back += (double)W2[h * MLP_H + g] * dz2[h];
+      dz1[g] = back * (1.0 - h1[g] * h1[g]);
+    }
+    for (int h = 0; h < MLP_H; ++h) {
+      for (int f = 0; f < MLP_F; ++f)
+        gW1[h * MLP_F + f] += dz1[h] * x[f];
+      gb1[h] += dz1[h];
+      for (int g = 0; g < MLP_H; ++g)
+        gW2[h * MLP_H + g] += dz2[h] * h1[g];
+      gb2[h] += dz2[h];
+      gW3[h] += dy * h2[h];
+    }
+    gb3[0] += dy;
+  }
+  for (int q = 0; q < P; ++q)
+    grad[q] /= m;
+  if (loss)
+    *loss = total / m;
+  return 0;
+}
+
 int oracle_dense_linkage(const float *X, int n, int d, int metric, float p, int linkage,
                          double *tree) {
   if (n < 2)

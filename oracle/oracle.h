@@ -19,6 +19,9 @@ int oracle_hdbscan_tree(const float *X, int n, int d, int metric, float p, int k
 int oracle_dense_on_matrix(float *D, int n, int linkage, double *tree);
 int oracle_dense_linkage(const float *X, int n, int d, int metric, float p, int linkage,
                          double *tree);
+int oracle_mlp_forward(const float *p, const float *X, int m, double *y);
+int oracle_mlp_gradient(const float *p, const float *X, const float *t, int m, double *grad,
+                        double *loss);
 
 #ifdef __cplusplus
 }

@@ -62,7 +62,7 @@ def test_argument_codes_before_any_gpu_work():
     core = np.zeros(4, np.float32)
     assert L.cuhc_hdbscan_tree(_lib.float_ptr(f), 4, 2, 0, 2.0, 5, 1.0, 0,
                                _lib.double_ptr(tree), _lib.float_ptr(core)) == -2
-    assert L.cuhc_hdbscan_tree(_lib.float_ptr(f), 4, 2, 0, 2.0, 2, 1.0, 3,
+    assert L.cuhc_hdbscan_tree(_lib.float_ptr(f), 4, 2, 0, 2.0, 2, 1.0, 4,
                                _lib.double_ptr(tree), _lib.float_ptr(core)) == -4
     assert L.cuhc_set_pruning(3) == -4
     assert L.cuhc_set_precision(3) == -4

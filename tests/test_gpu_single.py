@@ -141,3 +141,66 @@ def test_device_entry_point_matches_host_entry():
     np.testing.assert_array_equal(tree, Zo)
     dev.free()
     hdev.free()
+
+
+def kruskal_tree(n, edges):
+    """Scipy-format tree of edges (w, u, v) taken in the given order (Kruskal)."""
+    parent = np.arange(n)
+    cid, size = np.arange(n), np.ones(n, np.int64)
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+    rows = []
+    for w, u, v in edges:
+        ru, rv = find(u), find(v)
+        if ru == rv:
+            continue
+        a, b = cid[ru], cid[rv]
+        rows.append([min(a, b), max(a, b), w, size[ru] + size[rv]])
+        parent[rv] = ru
+        size[ru] += size[rv]
+        cid[ru] = n + len(rows) - 1
+    return np.array(rows, dtype=np.float64)
+
+
+@pytest.mark.parametrize("pruning", ["never", "always"])
+def test_large_grid_every_weight_tied(pruning):
+    # 300 x 240 integer grid (72,000 points, above the size where the edge
+    # order is sorted on the GPU): every MST edge weighs exactly 1, so the tree
+    # is Kruskal over the unit edges in (u, v) order, which the keys (weight,
+    # smaller endpoint) alone do not fix.
+    W, H = 300, 240
+    g = np.stack(np.meshgrid(np.arange(W), np.arange(H)), axis=-1).reshape(-1, 2)
+    X = g.astype(np.float32)
+    n = len(X)
+    idx = np.arange(n).reshape(H, W)
+    right = np.stack([idx[:, :-1].ravel(), idx[:, 1:].ravel()], 1)
+    down = np.stack([idx[:-1, :].ravel(), idx[1:, :].ravel()], 1)
+    unit = np.concatenate([right, down])
+    unit = unit[np.lexsort((unit[:, 1], unit[:, 0]))]
+    expected = kruskal_tree(n, [(1.0, int(u), int(v)) for u, v in unit])
+    np.testing.assert_array_equal(run(X, pruning=pruning), expected)
+
+
+@pytest.mark.parametrize("pruning", ["never", "always"])
+def test_large_shuffled_line_with_tied_gaps(pruning):
+    # 80,000 shuffled points on a line at integer positions with gaps in
+    # [1, 50]: distances are exact, every non-neighbouring pair is strictly
+    # farther than both gaps between them, so the MST is the chain of
+    # neighbours; its order (gap, smaller index, larger index) has ~1,600 edges
+    # per tied weight.
+    rng = np.random.default_rng(17)
+    n = 80_000
+    pos = np.cumsum(rng.integers(1, 51, n)).astype(np.float32)
+    perm = rng.permutation(n)
+    X = np.empty((n, 1), np.float32)
+    X[perm, 0] = pos  # point perm[i] sits at the i-th position
+    u, v = perm[:-1], perm[1:]
+    lo, hi = np.minimum(u, v), np.maximum(u, v)
+    gap = np.diff(pos).astype(np.float64)
+    order = np.lexsort((hi, lo, gap))
+    expected = kruskal_tree(n, [(gap[i], int(lo[i]), int(hi[i])) for i in order])
+    np.testing.assert_array_equal(run(X, pruning=pruning), expected)

@@ -197,3 +197,25 @@ def test_oracle_single_linkage_c1_scale_against_scipy():
     ref = linkage(X.astype(np.float64), "single")
     assert_same_dendrogram(Z, ref, rtol=1e-6, atol=0)
     assert max_rel_height_error(Z, ref) < 1e-6
+
+
+def test_mlp_gradient_matches_finite_differences():
+    # Central differences of the oracle's own FP64 loss pin its chain rule: a
+    # dropped tanh' factor or a transposed W2 fails by orders of magnitude.
+    rng = np.random.default_rng(0)
+    p = rng.normal(scale=0.3, size=oracle.MLP_PARAMS).astype(np.float32)
+    X = rng.normal(size=(17, 4)).astype(np.float32)
+    t = rng.normal(size=17).astype(np.float32)
+    g, loss = oracle.mlp_gradient(p, X, t)
+    y = oracle.mlp_forward(p, X)
+    assert loss == pytest.approx(0.5 * np.mean((y - t.astype(np.float64)) ** 2), rel=1e-12)
+    # p is float32, so perturb in float32 steps large enough to be exact.
+    idx = rng.choice(oracle.MLP_PARAMS, size=60, replace=False)
+    for q in idx:
+        h = np.float32(1e-2) * max(1.0, abs(float(p[q])))
+        up, dn = p.copy(), p.copy()
+        up[q] += h
+        dn[q] -= h
+        fd = (oracle.mlp_gradient(up, X, t)[1] - oracle.mlp_gradient(dn, X, t)[1]) / \
+            (float(up[q]) - float(dn[q]))
+        assert g[q] == pytest.approx(fd, rel=2e-3, abs=1e-6), q
