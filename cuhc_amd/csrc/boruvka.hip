@@ -3,9 +3,10 @@
 // (src/hdbscan.cuh:353-391, src/capi.cu:68-119).
 //
 // Each round: the GPU finds, for every point, its nearest point in another
-// component (one all-pairs pass); the host takes each component's smallest
-// edge under the total order (weight, u, v) in original indices, unites, and
-// relabels. Every chosen edge is the minimum edge leaving its component under
+// component (one all-pairs pass), then contracts (contract.hip): each
+// component's smallest edge under the total order (weight, u, v) in original
+// indices joins it to its neighbour and the components are relabelled, all in
+// device memory. Every chosen edge is the minimum edge leaving its component under
 // a strict total order, so the result is the unique MST under that order:
 // independent of the pass's tile order, pruning, or point permutation.
 #include "mst.hpp"
@@ -57,85 +58,64 @@ int DevicePoints::adopt(const float *devicePoints, int n, int d, bool spatial) {
 }
 
 int boruvkaMst(const float *X, int n, int d, MetricSpec m, WeightSpec w, const MstOptions &opt,
-               std::vector<Edge> &edges) {
-  edges.clear();
-  if (n < 2)
-    return 0;
-  edges.reserve(n - 1);
-  std::vector<int> comp(n);
-  std::iota(comp.begin(), comp.end(), 0);
-  std::vector<unsigned long long> nearest(n);
-  UnionFind sets(n); // over original indices
-  auto origOf = [&](int p) { return opt.hostOrig ? opt.hostOrig[p] : p; };
-
-  DevArray<int> dComp;
-  DevArray<unsigned long long> dNearest;
+               Contraction &ct) {
   Status s;
+  s(ct.alloc(n));
+  if (!s.ok() || n < 2)
+    return s.code;
+  DevArray<int> dComp, dPosOf, dDone;
+  DevArray<unsigned long long> dNearest;
   s(dComp.alloc(n));
   s(dNearest.alloc(n));
+  if (opt.devOrig) {
+    s(dPosOf.alloc(n));
+    if (s.ok())
+      s(inverseOrder(opt.devOrig, n, dPosOf));
+  }
+  // Pruned passes reuse the previous round's rows that are still valid.
+  if (opt.prune)
+    s(dDone.alloc(n));
+  if (s.ok())
+    s(resetComponents(dComp, n));
   MixedPass *mixed = nullptr;
   if (s.ok() && opt.mixed) {
     int st = 0;
     mixed = mixedPassCreate(X, n, d, &st);
     s(st);
   }
-
+  const int *posOf = opt.devOrig ? dPosOf.get() : nullptr;
   int numComponents = n;
   bool pruneThisPass = opt.prune;
-  std::vector<Edge> shortest;
-  std::vector<int> label(n);
+  bool haveResult = false; // dNearest holds the last round's exact result
   while (s.ok() && numComponents > 1) {
     if (mixed) {
-      s(mixedPassRun(mixed, comp.data(), numComponents, m.id == kSqEuclidean,
-                     numComponents == n ? nullptr : nearest.data(), dNearest));
+      s(mixedPassRun(mixed, dComp, numComponents, m.id == kSqEuclidean, !haveResult, dNearest));
     } else {
-      s(upload(dComp, comp.data(), n * sizeof(int)));
-      s(fill(dNearest, 0xff, n * sizeof(unsigned long long)));
+      const bool reuse = pruneThisPass && haveResult;
+      if (reuse)
+        s(markDoneRows(dNearest, dComp, posOf, n, dDone));
+      else
+        s(fill(dNearest, 0xff, n * sizeof(unsigned long long)));
       if (!s.ok())
         break;
       if (pruneThisPass) {
         double share = 1.0;
-        s(nearestOtherWalk(X, n, d, m, w, dComp, opt.devOrig, dNearest, &share, opt.stats));
+        s(nearestOtherWalk(X, n, d, m, w, dComp, opt.devOrig, reuse ? dDone.get() : nullptr,
+                           dNearest, &share, opt.stats));
         pruneThisPass = share <= kPruneGiveUpShare;
       } else {
         s(nearestOtherSymmetric(X, n, d, m, w, dComp, opt.devOrig, dNearest));
       }
     }
-    s(download(nearest.data(), dNearest, n * sizeof(unsigned long long)));
     if (!s.ok())
       break;
-    {
-      PhaseTimer timer(kPhaseContract);
-      // Shortest edge leaving each component, in original indices (every
-      // pass packs the other endpoint's original index).
-      shortest.assign(numComponents, Edge{-1, -1, 0.0});
-      for (int p = 0; p < n; ++p) {
-        if (nearest[p] == kEmptyKey)
-          continue;
-        const int a = origOf(p), b = keyIndex(nearest[p]);
-        const Edge e{std::min(a, b), std::max(a, b), (double)keyWeight(nearest[p])};
-        Edge &best = shortest[comp[p]];
-        if (best.u < 0 || edgeLess(e, best))
-          best = e;
-      }
-      const size_t before = edges.size();
-      for (const Edge &e : shortest)
-        if (e.u >= 0 && sets.unite(e.u, e.v) >= 0)
-          edges.push_back(e);
-      if (edges.size() == before) {
-        s(kErrDisconnected);
-        break;
-      }
-      // Relabel components 0..k-1 in order of first appearance by position.
-      std::fill(label.begin(), label.end(), -1);
-      numComponents = 0;
-      for (int p = 0; p < n; ++p) {
-        const int root = sets.find(origOf(p));
-        if (label[root] < 0)
-          label[root] = numComponents++;
-        comp[p] = label[root];
-      }
-    }
+    haveResult = true;
+    PhaseTimer timer(kPhaseContract);
+    int next = numComponents, added = 0;
+    s(contractRound(ct, dNearest, dComp, opt.devOrig, posOf, numComponents, &next, &added));
+    if (s.ok() && added == 0)
+      s(kErrDisconnected);
+    numComponents = next;
   }
   if (mixed)
     mixedPassDestroy(mixed);
@@ -146,15 +126,14 @@ int singleLinkageMst(const DevicePoints &pts, int n, int d, MetricSpec m, bool p
                      PassStats *stats, double *tree) {
   MstOptions opt;
   opt.prune = prune && pts.ordered();
-  opt.hostOrig = pts.ordered() ? pts.order.data() : nullptr;
   opt.devOrig = pts.ordered() ? pts.orig.get() : nullptr;
   opt.stats = stats;
   opt.mixed = mixed && !pts.ordered() && (m.id == kEuclidean || m.id == kSqEuclidean) &&
               mixedPassSupports(d);
-  std::vector<Edge> edges;
-  int status = boruvkaMst(pts.data(), n, d, m, WeightSpec{}, opt, edges);
+  Contraction ct;
+  int status = boruvkaMst(pts.data(), n, d, m, WeightSpec{}, opt, ct);
   if (status == 0)
-    status = treeFromMst(std::move(edges), n, tree);
+    status = treeFromDeviceEdges(ct.edges, n, tree);
   return status;
 }
 
@@ -168,16 +147,15 @@ int hdbscanTree(const DevicePoints &pts, int n, int d, MetricSpec m, int minSamp
                     prune && pts.ordered()));
   MstOptions opt;
   opt.prune = prune && pts.ordered();
-  opt.hostOrig = pts.ordered() ? pts.order.data() : nullptr;
   opt.devOrig = pts.ordered() ? pts.orig.get() : nullptr;
   opt.stats = stats;
   WeightSpec w;
   w.kind = kWeightMReach;
   w.core = core;
   w.alpha = alpha;
-  std::vector<Edge> edges;
+  Contraction ct;
   if (s.ok())
-    s(boruvkaMst(pts.data(), n, d, m, w, opt, edges));
+    s(boruvkaMst(pts.data(), n, d, m, w, opt, ct));
   if (s.ok() && hostCore) {
     std::vector<float> c(n);
     s(download(c.data(), core, n * sizeof(float)));
@@ -185,7 +163,7 @@ int hdbscanTree(const DevicePoints &pts, int n, int d, MetricSpec m, int minSamp
       hostCore[pts.ordered() ? pts.order[i] : i] = c[i];
   }
   if (s.ok())
-    s(treeFromMst(std::move(edges), n, tree));
+    s(treeFromDeviceEdges(ct.edges, n, tree));
   return s.code;
 }
 

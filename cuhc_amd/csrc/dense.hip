@@ -85,6 +85,7 @@ struct DenseState {
   int *itemCount;                 // [2]
   int itemCapacity;
   double *tree; // (n - 1) x 4, device
+  unsigned long long *clocks; // null, or [3]: device-clock ticks in selection, update, rescan
 };
 
 DEV unsigned long long blockMinKey(unsigned long long v, unsigned long long *red) {
@@ -149,9 +150,13 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
   const int per = (n + G - 1) / G;
   const int mine0 = blockIdx.x * per, mine1 = min(n, mine0 + per);
   const int gtid = blockIdx.x * kThreads + threadIdx.x, gthreads = G * kThreads;
+  const bool clocked = s.clocks && gtid == 0;
+  unsigned long long tSelect = 0, tUpdate = 0, tRescan = 0, t0 = 0, t1 = 0, t2 = 0;
 
   for (int step = 0; step < n - 1; ++step) {
     const int p = step & 1, q = p ^ 1;
+    if (clocked)
+      t0 = wall_clock64();
     // A: fold the previous step's rescans into fn, then this workgroup's minimum.
     if (step > 0) {
       const int cnt = min(s.itemCount[q], s.itemCapacity);
@@ -193,6 +198,10 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
       shIdB = b >= 0 ? s.id[b] : -1;
     }
     __syncthreads();
+    if (clocked) {
+      t1 = wall_clock64();
+      tSelect += t1 - t0;
+    }
     const int a = shA, b = shB, na = shNa, nb = shNb;
     const float dab = shDist;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -243,6 +252,10 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
       enqueueRow(s, p, a);
     }
     grid.sync();
+    if (clocked) {
+      t2 = wall_clock64();
+      tUpdate += t2 - t1;
+    }
 
     // D: rescans; bookkeeping of the merged slot.
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -257,6 +270,13 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
         atomicMin(&s.scratchKey[it.x], k);
     }
     grid.sync();
+    if (clocked)
+      tRescan += wall_clock64() - t2;
+  }
+  if (clocked) {
+    s.clocks[0] = tSelect;
+    s.clocks[1] = tUpdate;
+    s.clocks[2] = tRescan;
   }
 }
 
@@ -298,6 +318,7 @@ int denseLinkageOnCondensed(float *D, int n, int linkage, const int *initialSize
   DevArray<unsigned long long> fnKey, scratchKey, blockBest;
   DevArray<int2> items0, items1, initItems;
   DevArray<double> tree;
+  DevArray<unsigned long long> clocks;
   Status s;
   const int maxChunks = (n - 1 + kChunk - 1) / kChunk;
   const long long initCount = (long long)(n - 1) * maxChunks;
@@ -332,6 +353,12 @@ int denseLinkageOnCondensed(float *D, int n, int linkage, const int *initialSize
   st.itemCount = itemCount;
   st.itemCapacity = (int)cap;
   st.tree = tree;
+  st.clocks = nullptr;
+  if (timingEnabled()) {
+    s(clocks.alloc(3));
+    s(fill(clocks, 0, 3 * sizeof(unsigned long long)));
+    st.clocks = clocks;
+  }
   s(fill(itemCount, 0, 2 * sizeof(int)));
   PhaseTimer timer(kPhaseDense);
   hipLaunchKernelGGL(denseSetupKernel, dim3(std::min(1024, (n + 255) / 256)), dim3(256), 0,
@@ -349,6 +376,13 @@ int denseLinkageOnCondensed(float *D, int n, int linkage, const int *initialSize
                                       args, 0, stream()));
   }
   s(sync());
+  if (s.ok() && st.clocks) {
+    unsigned long long ticks[3];
+    s(download(ticks, clocks, sizeof(ticks)));
+    const double khz = wallClockKHz();
+    for (int i = 0; i < 3; ++i)
+      addMs(kPhaseDenseSelect + i, (double)ticks[i] / khz, 1);
+  }
   if (s.ok())
     s(download(hostTree, tree, (size_t)(n - 1) * 4 * sizeof(double)));
   if (s.ok())

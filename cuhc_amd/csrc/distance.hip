@@ -274,18 +274,28 @@ DEV int walkTile(int rowTile, int step, int numTiles) {
   return (tile >= 0 && tile < numTiles) ? tile : -1;
 }
 
+// One workgroup per row tile walks the column tiles outward from its own
+// (walkTile order). Lower bounds of the weight between the two boxes are
+// computed 64 steps at a time by wave 0, one step per lane, into LDS; a step
+// is skipped when its bound exceeds the largest current best of the tile's
+// open rows. Rows marked done (their nearest from the previous round is still
+// in another component, so still their nearest) are not open: they neither
+// need tiles nor hold the bound up, and a tile of done rows returns at once.
 template <class T, class Metric, class Weight>
 __global__ void __launch_bounds__(256)
     nearestOtherWalkKernel(const float *__restrict__ X, int n, int d, Metric metric,
                            Weight weight, const int *__restrict__ comp,
                            const int *__restrict__ orig, Boxes boxes,
+                           const int *__restrict__ done,
                            unsigned long long *__restrict__ nearest,
                            unsigned long long *tilesComputed, int rowTileStride) {
   constexpr int TR = T::TR;
+  constexpr int kBatch = 64;
   __shared__ __align__(16) TileSmem<T> sm;
   __shared__ unsigned long long rowBest[T::DIM];
+  __shared__ float lowerBound[kBatch];
   __shared__ float bound;
-  __shared__ int skip;
+  __shared__ int openRows;
   extern __shared__ float rowBox[]; // low[d], high[d]
 
   const int rowTile = blockIdx.x * rowTileStride, numTiles = boxes.numTiles;
@@ -296,9 +306,21 @@ __global__ void __launch_bounds__(256)
     rowBox[k] = boxes.low[(size_t)rowTile * d + k];
     rowBox[d + k] = boxes.high[(size_t)rowTile * d + k];
   }
-  if (threadIdx.x == 0)
-    bound = INFINITY;
+  // Wave 0 owns rows lane and lane + 64 for the bound reduction.
+  bool open0 = false, open1 = false;
+  if (threadIdx.x < 64) {
+    const int a = row0 + threadIdx.x, b = a + 64;
+    open0 = a < n && !(done && done[a]);
+    open1 = b < n && !(done && done[b]);
+    const unsigned long long any = __ballot(open0 || open1);
+    if (threadIdx.x == 0) {
+      bound = INFINITY;
+      openRows = any != 0;
+    }
+  }
   const bool rowTilePure = boxes.minComp[rowTile] == boxes.maxComp[rowTile];
+  const int rowCompTile = boxes.minComp[rowTile];
+  const float rowMinValue = boxes.minValue[rowTile];
   const float slack = 1.0f - boxSlack(d);
 
   int rowComp[TR];
@@ -311,80 +333,114 @@ __global__ void __launch_bounds__(256)
   }
   unsigned long long computed = 0;
   __syncthreads();
+  if (!openRows)
+    return; // uniform
 
-  for (int step = 0; step < 2 * numTiles; ++step) {
-    const int colTile = walkTile(rowTile, step, numTiles);
-    if (colTile < 0)
-      continue; // uniform across the workgroup
-    if (threadIdx.x == 0) {
-      bool canSkip = rowTilePure && boxes.minComp[colTile] == boxes.maxComp[colTile] &&
-                     boxes.minComp[colTile] == boxes.minComp[rowTile];
-      if (!canSkip && bound != INFINITY) {
+  const int steps = 2 * numTiles;
+  for (int base = 0; base < steps; base += kBatch) {
+    if (threadIdx.x < kBatch) {
+      // -1: never computed (outside the grid, or both tiles wholly one component).
+      const int step = base + (int)threadIdx.x;
+      const int colTile = step < steps ? walkTile(rowTile, step, numTiles) : -1;
+      float lb = -1.0f;
+      if (colTile >= 0 &&
+          !(rowTilePure && boxes.minComp[colTile] == boxes.maxComp[colTile] &&
+            boxes.minComp[colTile] == rowCompTile)) {
         const float db = boxLowerBound(metric, rowBox, rowBox + d, boxes.low + (size_t)colTile * d,
                                        boxes.high + (size_t)colTile * d, d);
-        const float wb = weight(db, boxes.minValue[rowTile], boxes.minValue[colTile]) * slack;
-        canSkip = wb > bound;
+        lb = weight(db, rowMinValue, boxes.minValue[colTile]) * slack;
       }
-      skip = canSkip;
+      lowerBound[threadIdx.x] = lb;
     }
     __syncthreads();
-    const bool doSkip = skip;
-    __syncthreads(); // everyone has read skip
-    if (doSkip)
-      continue;
-    ++computed;
-
-    const int col0 = colTile * T::DIM;
-    float w[TR][TR];
-    computeTile<T>(sm, X, n, X, n, d, metric, row0, col0, w);
-    int colComp[TR];
-    typename Weight::Value colVal[TR];
-    unsigned int colIdx[TR];
-#pragma unroll
-    for (int c = 0; c < TR; ++c) {
-      const int col = col0 + T::colInTile(c);
-      colComp[c] = col < n ? comp[col] : -1;
-      colVal[c] = weight.value(col < n ? col : 0);
-      colIdx[c] = col < n ? (unsigned)orig[col] : 0u;
-    }
-#pragma unroll
-    for (int r = 0; r < TR; ++r) {
-      unsigned long long best = kEmptyKey;
+    for (int j = 0; j < kBatch; ++j) {
+      const float lb = lowerBound[j];
+      if (lb < 0.0f || lb > bound)
+        continue; // uniform: lowerBound and bound are shared
+      const int colTile = walkTile(rowTile, base + j, numTiles);
+      ++computed;
+      const int col0 = colTile * T::DIM;
+      float w[TR][TR];
+      computeTile<T>(sm, X, n, X, n, d, metric, row0, col0, w);
+      int colComp[TR];
+      typename Weight::Value colVal[TR];
+      unsigned int colIdx[TR];
 #pragma unroll
       for (int c = 0; c < TR; ++c) {
-        const float ww = weight(w[r][c], rowVal[r], colVal[c]);
-        const bool counts = rowComp[r] >= 0 && colComp[c] >= 0 && colComp[c] != rowComp[r] &&
-                            ww != INFINITY;
-        if (counts)
-          best = minKey(best, packKey(ww, colIdx[c]));
+        const int col = col0 + T::colInTile(c);
+        colComp[c] = col < n ? comp[col] : -1;
+        colVal[c] = weight.value(col < n ? col : 0);
+        colIdx[c] = col < n ? (unsigned)orig[col] : 0u;
       }
-      best = rowGroupMin(best);
-      if (T::tc() == 0 && best != kEmptyKey) {
-        const int i = T::rowInTile(r);
-        rowBest[i] = minKey(rowBest[i], best);
-      }
-    }
-    __syncthreads();
-    // New bound: the largest current best over the valid rows (inf while a
-    // row has none yet). Wave 0 reduces the DIM values.
-    if (threadIdx.x < 64) {
-      float largest = 0.0f;
-      for (int i = threadIdx.x; i < T::DIM; i += 64)
-        if (row0 + i < n)
-          largest = fmaxf(largest, rowBest[i] == kEmptyKey ? INFINITY : keyWeight(rowBest[i]));
 #pragma unroll
-      for (int m = 32; m > 0; m >>= 1)
-        largest = fmaxf(largest, __shfl_xor(largest, m, 64));
-      if (threadIdx.x == 0)
-        bound = largest;
+      for (int r = 0; r < TR; ++r) {
+        unsigned long long best = kEmptyKey;
+#pragma unroll
+        for (int c = 0; c < TR; ++c) {
+          const float ww = weight(w[r][c], rowVal[r], colVal[c]);
+          const bool counts = rowComp[r] >= 0 && colComp[c] >= 0 && colComp[c] != rowComp[r] &&
+                              ww != INFINITY;
+          if (counts)
+            best = minKey(best, packKey(ww, colIdx[c]));
+        }
+        best = rowGroupMin(best);
+        if (T::tc() == 0 && best != kEmptyKey) {
+          const int i = T::rowInTile(r);
+          rowBest[i] = minKey(rowBest[i], best);
+        }
+      }
+      __syncthreads();
+      // New bound: the largest current best over the open rows (inf while
+      // one has none yet).
+      if (threadIdx.x < 64) {
+        float largest = 0.0f;
+        if (open0)
+          largest = rowBest[threadIdx.x] == kEmptyKey ? INFINITY : keyWeight(rowBest[threadIdx.x]);
+        if (open1)
+          largest = fmaxf(largest, rowBest[threadIdx.x + 64] == kEmptyKey
+                                       ? INFINITY
+                                       : keyWeight(rowBest[threadIdx.x + 64]));
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1)
+          largest = fmaxf(largest, __shfl_xor(largest, m, 64));
+        if (threadIdx.x == 0)
+          bound = largest;
+      }
+      __syncthreads();
     }
-    __syncthreads();
+    __syncthreads(); // lowerBound is rewritten by the next batch
   }
   for (int i = threadIdx.x; i < T::DIM; i += T::THREADS)
     if (row0 + i < n && rowBest[i] != kEmptyKey)
       atomicMin(&nearest[row0 + i], rowBest[i]);
   if (tilesComputed && threadIdx.x == 0)
     atomicAdd(tilesComputed, computed);
+}
+
+// Round r + 1 of Boruvka from round r's result: a point whose nearest (in
+// another component then) is still in another component keeps it (the other
+// points only became fewer); the rest start empty.
+__global__ void markDoneKernel(unsigned long long *__restrict__ nearest,
+                               const int *__restrict__ comp, const int *__restrict__ posOf, int n,
+                               int *__restrict__ done) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n)
+    return;
+  const unsigned long long key = nearest[p];
+  bool keep = false;
+  if (key != kEmptyKey) {
+    const int q = (int)keyIndex(key);
+    keep = comp[posOf ? posOf[q] : q] != comp[p];
+  }
+  done[p] = keep;
+  if (!keep)
+    nearest[p] = kEmptyKey;
+}
+
+__global__ void inverseOrderKernel(const int *__restrict__ orig, int n, int *__restrict__ posOf) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n)
+    posOf[orig[p]] = p;
 }
 
 __global__ void gatherRowsKernel(const float *__restrict__ src, const int *__restrict__ order,
@@ -559,8 +615,8 @@ struct DeviceBoxes {
 
 template <class T, class Metric, class Weight>
 int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &weight,
-            const int *comp, const int *orig, unsigned long long *nearest, double *share,
-            PassStats *stats, int stride) {
+            const int *comp, const int *orig, const int *done, unsigned long long *nearest,
+            double *share, PassStats *stats, int stride) {
   const int tiles = (n + T::DIM - 1) / T::DIM;
   DeviceBoxes boxes;
   DevArray<unsigned long long> counter;
@@ -578,7 +634,7 @@ int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &we
     PhaseTimer timer(kPhaseDistanceTiles);
     hipLaunchKernelGGL((nearestOtherWalkKernel<T, Metric, Weight>), dim3(sampled), dim3(256),
                        2 * d * sizeof(float), stream(), X, n, d, metric, weight, comp, orig,
-                       boxes.view, nearest, counter.get(), stride);
+                       boxes.view, done, nearest, counter.get(), stride);
     s((int)hipGetLastError());
   }
   unsigned long long computed = 0;
@@ -596,8 +652,8 @@ int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &we
 } // namespace
 
 int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, const int *comp,
-                     const int *orig, unsigned long long *nearest, double *share,
-                     PassStats *stats, int stride) {
+                     const int *orig, const int *done, unsigned long long *nearest,
+                     double *share, PassStats *stats, int stride) {
   if (!metricHasBoxBound(m.id))
     return kErrInvalidMetric;
   if (w.kind == kWeightWard) {
@@ -605,8 +661,8 @@ int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, c
       return kErrInvalidMetric;
     return withFill(d, [&](auto fill) {
       using T = Tile<8, decltype(fill)::value>;
-      return runWalk<T>(X, n, d, SqEuclidean{}, WardWeight{w.size}, comp, orig, nearest, share,
-                        stats, stride);
+      return runWalk<T>(X, n, d, SqEuclidean{}, WardWeight{w.size}, comp, orig, done, nearest,
+                        share, stats, stride);
     });
   }
   return withMetric(m, [&](auto metric) {
@@ -617,11 +673,25 @@ int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, c
       return withWeightFor(metric, w, [&](auto weight) {
         return withFill(d, [&](auto fill) {
           using T = Tile<8, decltype(fill)::value>;
-          return runWalk<T>(X, n, d, metric, weight, comp, orig, nearest, share, stats, stride);
+          return runWalk<T>(X, n, d, metric, weight, comp, orig, done, nearest, share, stats,
+                            stride);
         });
       });
     }
   });
+}
+
+int markDoneRows(unsigned long long *nearest, const int *comp, const int *posOf, int n,
+                 int *done) {
+  hipLaunchKernelGGL(markDoneKernel, dim3((n + 255) / 256), dim3(256), 0, stream(), nearest, comp,
+                     posOf, n, done);
+  return (int)hipGetLastError();
+}
+
+int inverseOrder(const int *orig, int n, int *posOf) {
+  hipLaunchKernelGGL(inverseOrderKernel, dim3((n + 255) / 256), dim3(256), 0, stream(), orig, n,
+                     posOf);
+  return (int)hipGetLastError();
 }
 
 int estimatePrunedShare(const float *X, int n, int d, MetricSpec m, double *share) {
@@ -641,7 +711,8 @@ int estimatePrunedShare(const float *X, int n, int d, MetricSpec m, double *shar
   hipLaunchKernelGGL(iotaKernel, dim3(std::min(1024, (n + 255) / 256)), dim3(256), 0, stream(),
                      orig.get(), n);
   s(fill(nearest, 0xff, nearest.bytes()));
-  s(nearestOtherWalk(X, n, d, m, WeightSpec{}, comp, orig, nearest, share, nullptr, stride));
+  s(nearestOtherWalk(X, n, d, m, WeightSpec{}, comp, orig, nullptr, nearest, share, nullptr,
+                     stride));
   return s.code;
 }
 

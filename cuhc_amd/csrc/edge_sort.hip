@@ -3,13 +3,11 @@
 // which is a radix sort of (weight bits, smaller endpoint) keys here instead
 // of a comparison sort on the host. Equal keys (two edges of one point with
 // the same weight) are ordered by the larger endpoint on the host afterwards.
-#include "host_tree.hpp"
-#include "runtime.hpp"
+#include "mst.hpp"
 
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
-#include <cstring>
 
 namespace cuhc {
 
@@ -20,102 +18,96 @@ struct PackedEdge {
   float w;
 };
 
-__global__ void edgeKeysKernel(const PackedEdge *__restrict__ edges, long long m, int uBits,
-                               unsigned long long *__restrict__ keys,
+__global__ void edgeKeysKernel(const int *__restrict__ u, const float *__restrict__ w, int m,
+                               int uBits, unsigned long long *__restrict__ keys,
                                unsigned int *__restrict__ idx) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m)
     return;
-  // Non-negative float bits order like the floats; +0 for -0.
-  const float w = edges[i].w == 0.0f ? 0.0f : edges[i].w;
-  keys[i] = (unsigned long long)__float_as_uint(w) << uBits | (unsigned int)edges[i].u;
+  // Weights are non-negative floats (packed keys clamp): their bits order
+  // like the values; +0 stands for -0.
+  const float wi = w[i] == 0.0f ? 0.0f : w[i];
+  keys[i] = (unsigned long long)__float_as_uint(wi) << uBits | (unsigned int)u[i];
   idx[i] = (unsigned int)i;
 }
 
-__global__ void edgeGatherKernel(const PackedEdge *__restrict__ edges,
-                                 const unsigned int *__restrict__ idx, long long m,
-                                 PackedEdge *__restrict__ out) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m)
-    out[i] = edges[idx[i]];
+__global__ void edgeGatherKernel(const int *__restrict__ u, const int *__restrict__ v,
+                                 const float *__restrict__ w, const unsigned int *__restrict__ idx,
+                                 int m, PackedEdge *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) {
+    const unsigned int j = idx[i];
+    out[i] = PackedEdge{u[j], v[j], w[j]};
+  }
 }
 
 } // namespace
 
-int sortEdgesGpu(std::vector<Edge> &edges, bool &done) {
-  done = false;
-  const long long m = (long long)edges.size();
-  if (m < 2 || m > 0x7fffffffLL)
+int treeFromDeviceEdges(const DeviceEdges &edges, int n, double *tree) {
+  const int m = edges.count;
+  if (m <= 0)
     return 0;
-  std::vector<PackedEdge> host(m);
-  int uBits = 1;
-  for (long long i = 0; i < m; ++i) {
-    const Edge &e = edges[i];
-    const float f = (float)e.w;
-    if (!(f >= 0.0f) || (double)f != e.w)
-      return 0; // not float32 weights: the caller sorts on the host
-    host[i] = PackedEdge{e.u, e.v, f};
-    while ((unsigned)e.u >> uBits)
-      ++uBits;
-  }
-  DevArray<PackedEdge> dIn, dOut;
-  DevArray<unsigned long long> keys, keysOut;
-  DevArray<unsigned int> idx, idxOut;
-  DevArray<unsigned char> temp;
+  std::vector<Edge> host(m);
   Status s;
-  s(dIn.alloc(m));
-  s(dOut.alloc(m));
-  s(keys.alloc(m));
-  s(keysOut.alloc(m));
-  s(idx.alloc(m));
-  s(idxOut.alloc(m));
-  size_t tempBytes = 0;
-  s((int)hipcub::DeviceRadixSort::SortPairs(nullptr, tempBytes, (unsigned long long *)nullptr,
-                                            (unsigned long long *)nullptr,
-                                            (unsigned int *)nullptr, (unsigned int *)nullptr,
-                                            (int)m, 0, 32 + uBits, stream()));
-  s(temp.alloc(std::max<size_t>(tempBytes, 1)));
-  s(upload(dIn, host.data(), m * sizeof(PackedEdge)));
-  if (!s.ok())
-    return s.code;
-  const int blocks = (int)((m + 255) / 256);
-  hipLaunchKernelGGL(edgeKeysKernel, dim3(blocks), dim3(256), 0, stream(), dIn.get(), m, uBits,
-                     keys.get(), idx.get());
-  s((int)hipGetLastError());
-  s((int)hipcub::DeviceRadixSort::SortPairs(temp.get(), tempBytes, keys.get(), keysOut.get(),
-                                            idx.get(), idxOut.get(), (int)m, 0, 32 + uBits,
-                                            stream()));
-  hipLaunchKernelGGL(edgeGatherKernel, dim3(blocks), dim3(256), 0, stream(), dIn.get(),
-                     idxOut.get(), m, dOut.get());
-  s((int)hipGetLastError());
-  s(download(host.data(), dOut, m * sizeof(PackedEdge)));
-  if (!s.ok())
-    return s.code;
-  for (long long i = 0; i < m; ++i)
-    edges[i] = Edge{host[i].u, host[i].v, (double)host[i].w};
-  // Runs of equal (weight, u): order by v.
-  for (long long i = 0; i < m;) {
-    long long j = i + 1;
-    while (j < m && edges[j].w == edges[i].w && edges[j].u == edges[i].u)
-      ++j;
-    if (j - i > 1)
-      std::sort(edges.begin() + i, edges.begin() + j, edgeLess);
-    i = j;
+  if (m < 65536) {
+    std::vector<int> hu(m), hv(m);
+    std::vector<float> hw(m);
+    s(download(hu.data(), edges.u, m * sizeof(int)));
+    s(download(hv.data(), edges.v, m * sizeof(int)));
+    s(download(hw.data(), edges.w, m * sizeof(float)));
+    if (!s.ok())
+      return s.code;
+    for (int i = 0; i < m; ++i)
+      host[i] = Edge{hu[i], hv[i], (double)hw[i]};
+    sortEdges(host);
+  } else {
+    int uBits = 1;
+    while (uBits < 31 && (n - 1) >> uBits)
+      ++uBits;
+    DevArray<unsigned long long> keys, keysOut;
+    DevArray<unsigned int> idx, idxOut;
+    DevArray<PackedEdge> packed;
+    DevArray<unsigned char> temp;
+    s(keys.alloc(m));
+    s(keysOut.alloc(m));
+    s(idx.alloc(m));
+    s(idxOut.alloc(m));
+    s(packed.alloc(m));
+    size_t tempBytes = 0;
+    s((int)hipcub::DeviceRadixSort::SortPairs(nullptr, tempBytes, (unsigned long long *)nullptr,
+                                              (unsigned long long *)nullptr,
+                                              (unsigned int *)nullptr, (unsigned int *)nullptr, m,
+                                              0, 32 + uBits, stream()));
+    s(temp.alloc(std::max<size_t>(tempBytes, 1)));
+    if (!s.ok())
+      return s.code;
+    const dim3 grid((m + 255) / 256);
+    hipLaunchKernelGGL(edgeKeysKernel, grid, dim3(256), 0, stream(), edges.u.get(),
+                       edges.w.get(), m, uBits, keys.get(), idx.get());
+    s((int)hipGetLastError());
+    s((int)hipcub::DeviceRadixSort::SortPairs(temp.get(), tempBytes, keys.get(), keysOut.get(),
+                                              idx.get(), idxOut.get(), m, 0, 32 + uBits,
+                                              stream()));
+    hipLaunchKernelGGL(edgeGatherKernel, grid, dim3(256), 0, stream(), edges.u.get(),
+                       edges.v.get(), edges.w.get(), idxOut.get(), m, packed.get());
+    s((int)hipGetLastError());
+    std::vector<PackedEdge> hp(m);
+    s(download(hp.data(), packed, m * sizeof(PackedEdge)));
+    if (!s.ok())
+      return s.code;
+    for (int i = 0; i < m; ++i)
+      host[i] = Edge{hp[i].u, hp[i].v, (double)hp[i].w};
+    // Runs of equal (weight, u): order by v.
+    for (int i = 0; i < m;) {
+      int j = i + 1;
+      while (j < m && host[j].w == host[i].w && host[j].u == host[i].u)
+        ++j;
+      if (j - i > 1)
+        std::sort(host.begin() + i, host.begin() + j, edgeLess);
+      i = j;
+    }
   }
-  done = true;
-  return 0;
-}
-
-int treeFromMst(std::vector<Edge> edges, int n, double *tree) {
-  bool sorted = false;
-  if (edges.size() >= 65536) {
-    const int status = sortEdgesGpu(edges, sorted);
-    if (status != 0)
-      return status;
-  }
-  if (!sorted)
-    sortEdges(edges);
-  treeFromOrderedMerges(edges, n, tree);
+  treeFromOrderedMerges(host, n, tree);
   return 0;
 }
 

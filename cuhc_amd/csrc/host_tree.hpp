@@ -52,15 +52,6 @@ void treeFromOrderedMerges(const std::vector<Edge> &merges, int n, double *tree)
 // Sorts by edgeLess on the host.
 void sortEdges(std::vector<Edge> &edges);
 
-// The same order on the GPU for float32 weights (a radix sort); done stays
-// false, and nothing is sorted, when the weights are not all float32 values.
-int sortEdgesGpu(std::vector<Edge> &edges, bool &done);
-
-// Single linkage from MST edges: GPU sort when there are many float32-weighted
-// edges, else the host sort; then assembly.
-int treeFromMst(std::vector<Edge> edges, int n, double *tree);
-
-
 // Makes heights non-decreasing up the tree (a parent at least its children),
 // then stable-sorts by height: merges in chain/round order -> tree order.
 void orderChainMerges(std::vector<Edge> &merges, int n);

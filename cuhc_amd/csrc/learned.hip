@@ -219,8 +219,8 @@ __global__ void __launch_bounds__(256)
   if (row >= n)
     return;
   const int c = count[row];
-  if (lane == 0)
-    atomicAdd(candidates, (unsigned long long)c);
+  if (lane == 0) // striped over 64 cache lines: one address would serialise a million waves
+    atomicAdd(&candidates[(row & 63) * 16], (unsigned long long)c);
   if (c < k || c > kCandPerRow) {
     if (lane == 0)
       fallback[atomicAdd(numFallback, 1)] = row;
@@ -299,7 +299,7 @@ int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, 
   s(count.alloc(n));
   s(fallback.alloc(n));
   s(numFallback.alloc(1));
-  s(counters.alloc(2));
+  s(counters.alloc(1 + 64 * 16)); // [0] tiles computed, then 64 striped candidate counts
   if (!s.ok())
     return s.code;
   PhaseTimer timer(kPhaseCore);
@@ -389,7 +389,7 @@ int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, 
                      (float)tsd, (float)(tm + margin), radius.get());
   s(cand.alloc((size_t)n * kCandPerRow));
   s(fill(numFallback, 0, sizeof(int)));
-  s(fill(counters, 0, 2 * sizeof(unsigned long long)));
+  s(fill(counters, 0, counters.bytes()));
   if (!s.ok())
     return s.code;
   s(withBoxMetric(m, [&](auto metric) {
@@ -411,13 +411,14 @@ int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, 
   s(download(&nf, numFallback, sizeof(int)));
   if (s.ok() && nf > 0)
     s(coreForRows(X, n, d, m, k, fallback, nf, core));
-  unsigned long long ctr[2] = {0, 0};
-  s(download(ctr, counters, sizeof(ctr)));
+  std::vector<unsigned long long> ctr(counters.count);
+  s(download(ctr.data(), counters, counters.bytes()));
   LearnedStats st;
   st.proposals = (unsigned long long)n;
   st.fallbacks = (unsigned long long)nf;
   st.hits = (unsigned long long)(n - nf);
-  st.candidates = ctr[1];
+  for (int k = 0; k < 64; ++k)
+    st.candidates += ctr[1 + k * 16];
   st.tiles = ctr[0];
   if (s.ok())
     learnedStatsAdd(st);

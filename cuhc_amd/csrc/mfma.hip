@@ -41,7 +41,10 @@
 // finite), which lets fminf map to v_min/v_min3 without canonicalisation.
 #include "mst.hpp"
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
+#include <cfloat>
 #include <atomic>
 #include <cmath>
 #include <numeric>
@@ -191,7 +194,7 @@ struct CandArgs {
   uint2 *cand;  // (numRows x 2 halves) x kCandCap: (half-block, block minimum bits)
   int *candCount; // numRows x 2
   float *theta;   // numRows
-  unsigned long long *recorded;
+  unsigned long long *recorded; // [0] entries recorded, [4] row-column pairs computed
 };
 
 __device__ __noinline__ float thetaOf(float m, float Ni, float Ei, float deltai, double sqrtKappa) {
@@ -259,15 +262,19 @@ DEV void blockEpilogue(f32x16 acc, RowLane &rl, bool anyMask, int j0, int lane, 
       if ((unsigned)(j0 + mOfReg(r, lane) - rl.cs) < span)
         acc[r] = INFINITY;
   }
-  float m = fminf(acc[0], acc[1]);
-#pragma unroll
-  for (int r = 2; r < 16; r += 2)
-    m = fminf(m, fminf(acc[r], acc[r + 1]));
+  // Minimum of the lane's 16 values as a tree of three-way minima.
+  const float m0 = fminf(fminf(acc[0], acc[1]), acc[2]);
+  const float m1 = fminf(fminf(acc[3], acc[4]), acc[5]);
+  const float m2 = fminf(fminf(acc[6], acc[7]), acc[8]);
+  const float m3 = fminf(fminf(acc[9], acc[10]), acc[11]);
+  const float m4 = fminf(fminf(acc[12], acc[13]), acc[14]);
+  const float m = fminf(fminf(fminf(m0, m1), m2), fminf(fminf(m3, m4), acc[15]));
   rl.runMin = fminf(rl.runMin, m);
   // Record this lane's 16 columns of the block when they can hold a candidate
   // (the lane's theta is never below the row's final one, so nothing needed
-  // is missed).
-  if (rl.slot >= 0 && m <= rl.th && m < INFINITY) {
+  // is missed). th is -inf for rows outside the range and FLT_MAX before the
+  // first tile end, so one comparison also drops wholly masked blocks.
+  if (m <= rl.th) {
     if (rl.count < kCandCap)
       a.cand[((size_t)rl.slot * 2 + hi) * kCandCap + rl.count] =
           make_uint2((unsigned)block, __float_as_uint(m));
@@ -324,7 +331,7 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
     rl[rb].D = (valid ? a.rho[orig] : 0.0f) + a.rhoMax;
     rl[rb].runMin = INFINITY;
     rl[rb].rowMin = INFINITY;
-    rl[rb].th = INFINITY;
+    rl[rb].th = valid ? FLT_MAX : -INFINITY;
     rl[rb].count = 0;
   }
 
@@ -349,6 +356,8 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
   };
 
   const int numTiles = a.nPad / kTileCols;
+  const int validRows = max(0, min(kWaveRows, a.numRows - rowBase));
+  unsigned long long computedTiles = 0;
   fetch(0);
   store(0);
   __syncthreads();
@@ -369,6 +378,7 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
     const bool skipAll = __ballot(!contained) == 0;
     const bool anyMask = __ballot(inter) != 0;
     if (!skipAll) {
+      ++computedTiles;
       BlockOps<KB> o0, o1;
       loadBlock<KB>(o0, sx[buf], sn[buf], 0, l32, hi);
 #pragma unroll 1
@@ -399,7 +409,7 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
       for (int rb = 0; rb < 4; ++rb) {
         const float m = fminf(rl[rb].runMin, __shfl_xor(rl[rb].runMin, 32, 64));
         rl[rb].runMin = m;
-        if (m < rl[rb].rowMin) {
+        if (m < rl[rb].rowMin && rl[rb].slot >= 0) {
           rl[rb].rowMin = m;
           rl[rb].th = thetaOf(m, rl[rb].N, rl[rb].E, rl[rb].D, sqrtKappa);
         }
@@ -423,13 +433,20 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
     rec += __shfl_xor(rec, m, 64);
   if (lane == 0 && rec)
     atomicAdd(a.recorded, rec);
+  if (lane == 0 && computedTiles && validRows)
+    atomicAdd(a.recorded + 4, computedTiles * validRows * kTileCols);
 }
 
 // ---------------------------------------------------------------------------
-// Exact FP32 recheck: one wave per row; 16 lanes per recorded half-block.
+// Exact FP32 recheck: one wave per row. The row's 64 list entries (32 per
+// half) are loaded at once, one per lane; the entries still within the row's
+// final theta are taken four at a time, 16 lanes per half-block, one column per
+// lane. Column points come from the permuted FP32 copy xp (the same values as
+// X), so no load waits on the permutation.
 
 struct RecheckArgs {
-  const float *X;
+  const float *X;  // n x d, original order
+  const float *xp; // n x d, permuted like the columns
   int n, d;
   const int *perm;
   const int *rows;
@@ -439,8 +456,14 @@ struct RecheckArgs {
   const float *theta;
   bool sqeuclidean;
   unsigned long long *nearest; // by original index
-  unsigned long long *stats;   // [1] half-blocks rechecked, [2] pairs, [3] overflow rows
+  // kStatStripes stripes of one 128-byte line each: [1] half-blocks rechecked,
+  // [2] pairs, [3] overflow rows (striped so that a million waves do not
+  // serialise on one address).
+  unsigned long long *stats;
 };
+
+constexpr int kStatStripes = 64;
+constexpr int kStripeWords = 16;
 
 // FP32 direct-form weight of (row in LDS, column position col): the plain fold.
 DEV unsigned long long recheckColumn(const RecheckArgs &a, const float *xi, int col, int cs,
@@ -448,19 +471,28 @@ DEV unsigned long long recheckColumn(const RecheckArgs &a, const float *xi, int 
   if (col >= a.n || (col >= cs && col < ce))
     return kEmptyKey;
   const int j = a.perm[col];
-  const float *xj = a.X + (size_t)j * a.d;
+  const float *xj = a.xp + (size_t)col * a.d;
   float acc = 0.0f;
   if ((a.d & 3) == 0) {
-    for (int k = 0; k < a.d; k += 4) {
-      const float4 v = *reinterpret_cast<const float4 *>(xj + k);
-      float diff = xi[k] - v.x;
-      acc = fmaf(diff, diff, acc);
-      diff = xi[k + 1] - v.y;
-      acc = fmaf(diff, diff, acc);
-      diff = xi[k + 2] - v.z;
-      acc = fmaf(diff, diff, acc);
-      diff = xi[k + 3] - v.w;
-      acc = fmaf(diff, diff, acc);
+    // Every load is issued before the fold uses it (d <= 64).
+    float4 v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (4 * q < a.d)
+        v[q] = *reinterpret_cast<const float4 *>(xj + 4 * q);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      if (4 * q < a.d) {
+        const int k = 4 * q;
+        float diff = xi[k] - v[q].x;
+        acc = fmaf(diff, diff, acc);
+        diff = xi[k + 1] - v[q].y;
+        acc = fmaf(diff, diff, acc);
+        diff = xi[k + 2] - v[q].z;
+        acc = fmaf(diff, diff, acc);
+        diff = xi[k + 3] - v[q].w;
+        acc = fmaf(diff, diff, acc);
+      }
     }
   } else {
     for (int k = 0; k < a.d; ++k) {
@@ -473,21 +505,24 @@ DEV unsigned long long recheckColumn(const RecheckArgs &a, const float *xi, int 
 }
 
 __global__ void __launch_bounds__(256) recheckKernel(RecheckArgs a) {
+  static_assert(kCandCap == 32, "one list entry per lane");
   __shared__ float rowX[4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = blockIdx.x * 4 + wave;
   if (r >= a.numRows)
     return; // wave-uniform; no block-wide barrier below
+  // Independent loads: the row, its counts, theta and every list slot (slots
+  // past a count hold stale data and are masked below).
   const int pos = a.rows[3 * r], cs = a.rows[3 * r + 1], ce = a.rows[3 * r + 2];
-  const int orig = a.perm[pos];
+  const int c0 = a.candCount[(size_t)r * 2], c1 = a.candCount[(size_t)r * 2 + 1];
+  const float th = a.theta[r];
+  const uint2 entry = a.cand[(size_t)r * 2 * kCandCap + lane];
   float *xi = rowX[wave];
   for (int k = lane; k < a.d; k += 64)
-    xi[k] = a.X[(size_t)orig * a.d + k];
+    xi[k] = a.xp[(size_t)pos * a.d + k];
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const int c0 = a.candCount[(size_t)r * 2], c1 = a.candCount[(size_t)r * 2 + 1];
-  const float th = a.theta[r];
   unsigned long long best = kEmptyKey;
   unsigned long long blocks = 0;
   const bool overflow = c0 > kCandCap || c1 > kCandCap;
@@ -495,43 +530,97 @@ __global__ void __launch_bounds__(256) recheckKernel(RecheckArgs a) {
     for (int col = lane; col < a.n; col += 64)
       best = minKey(best, recheckColumn(a, xi, col, cs, ce));
   } else {
-    // Entry e: list e / ..., 16 lanes each; half-block b = tile * 8 + cb * 2 + hi
-    // covers columns tile*128 + cb*32 + g*8 + hi*4 + q, g, q in 0..3.
-    const int total = c0 + c1;
+    const bool listed = lane < 32 ? lane < c0 : lane - 32 < c1;
+    unsigned long long live = __ballot(listed && __uint_as_float(entry.y) <= th);
+    blocks = (unsigned long long)__popcll(live);
     const int sub = lane >> 4, idx = lane & 15;
-    for (int e0 = 0; e0 < total; e0 += 4) {
-      const int e = e0 + sub;
-      unsigned long long k = kEmptyKey;
-      if (e < total) {
-        const uint2 c = e < c0 ? a.cand[((size_t)r * 2) * kCandCap + e]
-                               : a.cand[((size_t)r * 2 + 1) * kCandCap + (e - c0)];
-        if (__uint_as_float(c.y) <= th) {
-          const int blk = (int)c.x;
-          const int col = (blk >> 3) * kTileCols + ((blk >> 1) & 3) * 32 + (idx >> 2) * 8 +
-                          (blk & 1) * 4 + (idx & 3);
-          k = recheckColumn(a, xi, col, cs, ce);
-          if (idx == 0)
-            ++blocks;
-        }
+    while (live) {
+      // The next four live entries (wave-uniform); this lane's group takes one.
+      int pick[4];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        pick[g] = live ? __ffsll((long long)live) - 1 : -1;
+        live &= live ? live - 1 : 0ull;
       }
-      best = minKey(best, k);
+      const int e = sub == 0 ? pick[0] : sub == 1 ? pick[1] : sub == 2 ? pick[2] : pick[3];
+      const int blk = (int)__shfl(entry.x, e < 0 ? 0 : e, 64);
+      if (e >= 0) {
+        // Half-block b = tile * 8 + cb * 2 + hi covers columns
+        // tile*128 + cb*32 + g*8 + hi*4 + q, g, q in 0..3.
+        const int col = (blk >> 3) * kTileCols + ((blk >> 1) & 3) * 32 + (idx >> 2) * 8 +
+                        (blk & 1) * 4 + (idx & 3);
+        best = minKey(best, recheckColumn(a, xi, col, cs, ce));
+      }
     }
   }
 #pragma unroll
-  for (int m = 32; m > 0; m >>= 1) {
+  for (int m = 32; m > 0; m >>= 1)
     best = minKey(best, shflXorKey(best, m));
-    blocks += shflXorKey(blocks, m);
-  }
   if (lane == 0) {
-    a.nearest[orig] = best;
+    a.nearest[a.perm[pos]] = best;
+    unsigned long long *st = a.stats + (size_t)(r % kStatStripes) * kStripeWords;
     if (overflow) {
-      atomicAdd(&a.stats[3], 1ull);
-      atomicAdd(&a.stats[2], (unsigned long long)a.n);
+      atomicAdd(&st[3], 1ull);
+      atomicAdd(&st[2], (unsigned long long)a.n);
     } else if (blocks) {
-      atomicAdd(&a.stats[1], blocks);
-      atomicAdd(&a.stats[2], blocks * 16);
+      atomicAdd(&st[1], blocks);
+      atomicAdd(&st[2], blocks * 16);
     }
   }
+}
+
+// Device preparation of a round: positions 0..n-1, each component's range in
+// the sorted order, and the list of rows (position in the sorted order and the
+// row's component range) to search.
+__global__ void positionsKernel(int *out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    out[i] = i;
+}
+
+__global__ void componentRangesKernel(const int *__restrict__ sortedComp, int n, int *start,
+                                      int *end) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const int c = sortedComp[i];
+  if (i == 0 || sortedComp[i - 1] != c)
+    start[c] = i;
+  if (i == n - 1 || sortedComp[i + 1] != c)
+    end[c] = i + 1;
+}
+
+__global__ void staleRowsKernel(const int *__restrict__ perm, const int *__restrict__ comp,
+                                const unsigned long long *__restrict__ prev, int n,
+                                bool firstRound, int *flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n)
+    return;
+  if (i == n) {
+    flag[n] = 0;
+    return;
+  }
+  const int p = perm[i];
+  bool stale = true;
+  if (!firstRound) {
+    const unsigned long long key = prev[p];
+    stale = key == kEmptyKey || comp[(int)keyIndex(key)] == comp[p];
+  }
+  flag[i] = stale ? 1 : 0;
+}
+
+__global__ void rowListKernel(const int *__restrict__ perm, const int *__restrict__ comp,
+                              const int *__restrict__ flag, const int *__restrict__ index,
+                              const int *__restrict__ start, const int *__restrict__ end, int n,
+                              int *rows) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flag[i])
+    return;
+  const int c = comp[perm[i]];
+  const int r = index[i];
+  rows[3 * r] = i;
+  rows[3 * r + 1] = start[c];
+  rows[3 * r + 2] = end[c];
 }
 
 int checkLayout() {
@@ -562,12 +651,14 @@ struct MixedPass {
   int n = 0, d = 0, DP = 0, nPad = 0;
   const float *X = nullptr;
   DevArray<_Float16> xh, xhp;
-  DevArray<float> norm, normp, R, rho, mu, theta;
+  DevArray<float> norm, normp, R, rho, mu, theta, xp;
   DevArray<int> perm, rows, candCount;
+  DevArray<int> positions, sortedComp, compStart, compEnd, rowFlag, rowIndex;
+  DevArray<unsigned char> sortTemp, scanTemp;
+  size_t sortBytes = 0, scanBytes = 0;
   DevArray<uint2> cand;
-  DevArray<unsigned long long> stats;
+  DevArray<unsigned long long> stats, recheckStats;
   float Rmax = 0.0f, rhoMax = 0.0f;
-  std::vector<int> hPerm, hRows, start, cursor;
 };
 
 MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
@@ -596,12 +687,31 @@ MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
   s(mp->rows.alloc((size_t)3 * n));
   s(mp->candCount.alloc((size_t)2 * n));
   s(mp->theta.alloc(n));
+  s(mp->xp.alloc((size_t)n * d));
   s(mp->cand.alloc((size_t)2 * n * kCandCap));
-  s(mp->stats.alloc(4));
+  s(mp->stats.alloc(5));
+  s(mp->recheckStats.alloc((size_t)kStatStripes * kStripeWords));
+  s(mp->positions.alloc(n));
+  s(mp->sortedComp.alloc(n));
+  s(mp->compStart.alloc(n));
+  s(mp->compEnd.alloc(n));
+  s(mp->rowFlag.alloc(n + 1));
+  s(mp->rowIndex.alloc(n + 1));
+  s((int)hipcub::DeviceRadixSort::SortPairs(nullptr, mp->sortBytes, (unsigned int *)nullptr,
+                                            (unsigned int *)nullptr, (int *)nullptr,
+                                            (int *)nullptr, n, 0, 32, stream()));
+  s((int)hipcub::DeviceScan::ExclusiveSum(nullptr, mp->scanBytes, (int *)nullptr, (int *)nullptr,
+                                          n + 1, stream()));
+  s(mp->sortTemp.alloc(std::max<size_t>(mp->sortBytes, 1)));
+  s(mp->scanTemp.alloc(std::max<size_t>(mp->scanBytes, 1)));
+  if (s.ok())
+    hipLaunchKernelGGL(positionsKernel, dim3((n + 255) / 256), dim3(256), 0, stream(),
+                       mp->positions.get(), n);
   if (s.ok()) {
     s(fill(sums, 0, d * sizeof(double)));
     s(fill(maxes, 0, 3 * sizeof(unsigned int)));
-    s(fill(mp->stats, 0, 4 * sizeof(unsigned long long)));
+    s(fill(mp->stats, 0, 5 * sizeof(unsigned long long)));
+    s(fill(mp->recheckStats, 0, mp->recheckStats.bytes()));
     hipLaunchKernelGGL(columnSumsKernel, dim3(std::max(1, std::min(1024, n / 64 + 1))),
                        dim3(256), 0, stream(), X, n, d, sums.get());
     std::vector<double> hs(d);
@@ -632,56 +742,64 @@ MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
     delete mp;
     return nullptr;
   }
-  mp->hPerm.resize(n);
-  mp->hRows.resize((size_t)3 * n);
   return mp;
 }
 
 void mixedPassDestroy(MixedPass *mp) {
   if (!mp)
     return;
-  unsigned long long st[4] = {0, 0, 0, 0};
-  if (download(st, mp->stats, sizeof(st)) == 0)
+  unsigned long long st[5] = {0, 0, 0, 0, 0};
+  std::vector<unsigned long long> stripes((size_t)kStatStripes * kStripeWords);
+  if (download(stripes.data(), mp->recheckStats, mp->recheckStats.bytes()) == 0 &&
+      download(st, mp->stats, sizeof(st)) == 0) {
+    for (int k = 0; k < kStatStripes; ++k)
+      for (int i = 1; i < 4; ++i)
+        st[i] += stripes[(size_t)k * kStripeWords + i];
     for (int i = 0; i < 4; ++i)
       __atomic_fetch_add(&gStats[i], st[i], __ATOMIC_RELAXED);
+    addUnits(kPhaseMfmaCandidates, (long long)st[4]); // pairs the MFMA tiles computed
+  }
   delete mp;
 }
 
 int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeuclidean,
-                 const unsigned long long *prev, unsigned long long *nearest) {
+                 bool firstRound, unsigned long long *nearest) {
   const int n = mp->n;
-  // Component-contiguous order (by component, then original index).
-  mp->start.assign(numComponents + 1, 0);
-  for (int i = 0; i < n; ++i)
-    ++mp->start[comp[i] + 1];
-  for (int c = 0; c < numComponents; ++c)
-    mp->start[c + 1] += mp->start[c];
-  mp->cursor.assign(mp->start.begin(), mp->start.end() - 1);
-  for (int i = 0; i < n; ++i)
-    mp->hPerm[mp->cursor[comp[i]]++] = i;
-  // Rows to search: all in the first round; later only points whose previous
-  // nearest joined their own component.
-  int numRows = 0;
-  for (int p = 0; p < n; ++p) {
-    const int i = mp->hPerm[p];
-    const bool stale = prev == nullptr || prev[i] == kEmptyKey || comp[keyIndex(prev[i])] == comp[i];
-    if (!stale)
-      continue;
-    mp->hRows[3 * numRows] = p;
-    mp->hRows[3 * numRows + 1] = mp->start[comp[i]];
-    mp->hRows[3 * numRows + 2] = mp->start[comp[i] + 1];
-    ++numRows;
-  }
   Status s;
-  s((int)hipMemcpyAsync(mp->perm, mp->hPerm.data(), n * sizeof(int), hipMemcpyHostToDevice,
-                        stream()));
-  s((int)hipMemcpyAsync(mp->rows, mp->hRows.data(), (size_t)3 * numRows * sizeof(int),
-                        hipMemcpyHostToDevice, stream()));
+  // Component-contiguous order: a stable radix sort of the positions by
+  // component (so by component, then original index), and each component's
+  // range in it.
+  int bits = 1;
+  while (bits < 31 && (numComponents - 1) >> bits)
+    ++bits;
+  size_t bytes = mp->sortBytes;
+  // Component labels are non-negative: sorted as unsigned keys.
+  s((int)hipcub::DeviceRadixSort::SortPairs(
+      mp->sortTemp.get(), bytes, reinterpret_cast<const unsigned int *>(comp),
+      reinterpret_cast<unsigned int *>(mp->sortedComp.get()), mp->positions.get(),
+      mp->perm.get(), n, 0, bits, stream()));
+  const dim3 grid((n + 255) / 256);
+  hipLaunchKernelGGL(componentRangesKernel, grid, dim3(256), 0, stream(), mp->sortedComp.get(), n,
+                     mp->compStart.get(), mp->compEnd.get());
+  // Rows to search: all in the first round; later only points whose previous
+  // nearest joined their own component. Listed in component order.
+  hipLaunchKernelGGL(staleRowsKernel, dim3((n + 1 + 255) / 256), dim3(256), 0, stream(),
+                     mp->perm.get(), comp, nearest, n, firstRound, mp->rowFlag.get());
+  bytes = mp->scanBytes;
+  s((int)hipcub::DeviceScan::ExclusiveSum(mp->scanTemp.get(), bytes, mp->rowFlag.get(),
+                                          mp->rowIndex.get(), n + 1, stream()));
+  hipLaunchKernelGGL(rowListKernel, grid, dim3(256), 0, stream(), mp->perm.get(), comp,
+                     mp->rowFlag.get(), mp->rowIndex.get(), mp->compStart.get(),
+                     mp->compEnd.get(), n, mp->rows.get());
+  s((int)hipGetLastError());
+  int numRows = 0;
+  s(download(&numRows, mp->rowIndex.get() + n, sizeof(int)));
   if (!s.ok() || numRows == 0)
-    return s.code ? s.code : sync();
+    return s.code;
   hipLaunchKernelGGL(permuteHalfKernel, dim3(2048), dim3(256), 0, stream(), mp->xh.get(),
                      mp->norm.get(), mp->perm.get(), n, mp->nPad, mp->DP, mp->xhp.get(),
                      mp->normp.get());
+  s(gatherRows(mp->X, mp->perm, n, mp->d, mp->xp));
   CandArgs ca;
   ca.xhp = mp->xhp;
   ca.normp = mp->normp;
@@ -707,7 +825,6 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   const int blocks = (numRows + kBlockRows - 1) / kBlockRows;
   {
     PhaseTimer timer(kPhaseMfmaCandidates);
-    addUnits(kPhaseMfmaCandidates, (long long)numRows * n);
     switch (KB) {
     case 1: hipLaunchKernelGGL(mfmaCandidateKernel<1>, dim3(blocks), dim3(256), 0, stream(), ca); break;
     case 2: hipLaunchKernelGGL(mfmaCandidateKernel<2>, dim3(blocks), dim3(256), 0, stream(), ca); break;
@@ -718,6 +835,7 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   }
   RecheckArgs ra;
   ra.X = mp->X;
+  ra.xp = mp->xp;
   ra.n = n;
   ra.d = mp->d;
   ra.perm = mp->perm;
@@ -728,7 +846,7 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   ra.theta = mp->theta;
   ra.sqeuclidean = sqeuclidean;
   ra.nearest = nearest;
-  ra.stats = mp->stats;
+  ra.stats = mp->recheckStats;
   {
     PhaseTimer timer(kPhaseRecheck);
     hipLaunchKernelGGL(recheckKernel, dim3((numRows + 3) / 4), dim3(256), 0, stream(), ra);

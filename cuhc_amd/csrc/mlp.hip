@@ -2,9 +2,9 @@
 //
 // Forward/backward: one thread per sample keeps both hidden layers in
 // registers and writes the activations and back-propagated errors it needs
-// (h1, h2, dz1, dz2, dy) to a work buffer. Gradients: one thread per
-// parameter sums its products over the samples in order (deterministic, no
-// atomics). Adam: one thread per parameter.
+// (h1, h2, dz1, dz2, dy) to a work buffer. Gradients: FP64 sums over fixed
+// sample chunks, then over the chunks in order (deterministic, no atomics).
+// Adam: one thread per parameter.
 #include "mlp.hpp"
 #include "common.hpp"
 
@@ -103,44 +103,65 @@ __global__ void __launch_bounds__(256) backwardKernel(const float *__restrict__ 
   w[4 * H] = dy;
 }
 
-// One thread per parameter: sum over samples in order, divided by m.
-__global__ void gradientKernel(const float *__restrict__ X, const float *__restrict__ work, int m,
-                               float *__restrict__ grad, float *loss) {
+// Gradient in two deterministic stages. Stage 1: workgroup (g, c) sums, for
+// the 64 parameters of group g (one per lane), the products of samples chunk c
+// (kChunk samples, a quarter per wave, the waves' sums added in wave order) in
+// FP64 into partial[c][q]. Stage 2: one thread per parameter adds the chunks
+// in order and divides by m; thread P writes the loss (mean of 0.5 dy^2).
+constexpr int kChunk = 256;
+
+DEV double productOf(int q, const float *__restrict__ X, const float *__restrict__ w, int s) {
+  if (q < OB1) {
+    const int h = (q - OW1) / F, f = (q - OW1) % F;
+    return (double)w[2 * H + h] * X[(size_t)s * F + f];
+  }
+  if (q < OW2)
+    return w[2 * H + (q - OB1)];
+  if (q < OB2) {
+    const int h = (q - OW2) / H, g = (q - OW2) % H;
+    return (double)w[3 * H + h] * w[g];
+  }
+  if (q < OW3)
+    return w[3 * H + (q - OB2)];
+  if (q < OB3)
+    return (double)w[4 * H] * w[H + (q - OW3)];
+  if (q == OB3)
+    return w[4 * H];
+  return 0.5 * (double)w[4 * H] * w[4 * H]; // q == P: the loss term
+}
+
+__global__ void __launch_bounds__(256)
+    gradientPartialKernel(const float *__restrict__ X, const float *__restrict__ work, int m,
+                          double *__restrict__ partial) {
+  __shared__ double waveSum[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = blockIdx.x * 64 + lane; // 0..P (P = the loss)
+  const int chunk = blockIdx.y;
+  const int first = chunk * kChunk, last = min(m, first + kChunk);
+  const int per = kChunk / 4;
+  double acc = 0.0;
+  if (q <= P)
+    for (int s = first + wave * per; s < min(last, first + (wave + 1) * per); ++s)
+      acc += productOf(q, X, work + (size_t)s * WS, s);
+  waveSum[wave][lane] = acc;
+  __syncthreads();
+  if (wave == 0 && q <= P)
+    partial[(size_t)chunk * (P + 1) + q] =
+        ((waveSum[0][lane] + waveSum[1][lane]) + waveSum[2][lane]) + waveSum[3][lane];
+}
+
+__global__ void gradientFinishKernel(const double *__restrict__ partial, int chunks, int m,
+                                     float *__restrict__ grad, float *loss) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q > P)
     return;
-  double acc = 0.0; // FP64 sums: order-independent enough to compare with the oracle
-  if (q == P) {
-    if (loss) {
-      for (int s = 0; s < m; ++s) {
-        const double dy = work[(size_t)s * WS + 4 * H];
-        acc += 0.5 * dy * dy;
-      }
-      *loss = (float)(acc / m);
-    }
-    return;
-  }
-  for (int s = 0; s < m; ++s) {
-    const float *w = work + (size_t)s * WS;
-    double v;
-    if (q < OB1) {
-      const int h = (q - OW1) / F, f = (q - OW1) % F;
-      v = (double)w[2 * H + h] * X[(size_t)s * F + f];
-    } else if (q < OW2) {
-      v = w[2 * H + (q - OB1)];
-    } else if (q < OB2) {
-      const int h = (q - OW2) / H, g = (q - OW2) % H;
-      v = (double)w[3 * H + h] * w[g];
-    } else if (q < OW3) {
-      v = w[3 * H + (q - OB2)];
-    } else if (q < OB3) {
-      v = (double)w[4 * H] * w[H + (q - OW3)];
-    } else {
-      v = w[4 * H];
-    }
-    acc += v;
-  }
-  grad[q] = (float)(acc / m);
+  double acc = 0.0;
+  for (int c = 0; c < chunks; ++c)
+    acc += partial[(size_t)c * (P + 1) + q];
+  if (q < P)
+    grad[q] = (float)(acc / m);
+  else if (loss)
+    *loss = (float)(acc / m);
 }
 
 __global__ void adamKernel(float *p, const float *g, float *mom, float *vel, int step, float lr) {
@@ -157,14 +178,26 @@ __global__ void adamKernel(float *p, const float *g, float *mom, float *vel, int
 
 } // namespace
 
-size_t mlpWorkFloats(int m) { return (size_t)m * WS; }
+size_t mlpWorkFloats(int m) {
+  // per-sample records, then the FP64 chunk partials (P + 1 per chunk)
+  const size_t chunks = (size_t)(m + kChunk - 1) / kChunk;
+  return (size_t)m * WS + 2 * chunks * (P + 1) + 2;
+}
 
 int mlpGradient(const float *params, const float *X, const float *t, int m, float *work,
                 float *grad, float *loss) {
+  if (m <= 0)
+    return 0;
+  const int chunks = (m + kChunk - 1) / kChunk;
+  // 8-byte aligned start of the partials after the per-sample records
+  double *partial = reinterpret_cast<double *>(
+      work + (((size_t)m * WS + 1) & ~(size_t)1));
   hipLaunchKernelGGL(backwardKernel, dim3((m + 255) / 256), dim3(256), 0, stream(), params, X, t,
                      m, work);
-  hipLaunchKernelGGL(gradientKernel, dim3((P + 1 + 255) / 256), dim3(256), 0, stream(), X, work,
-                     m, grad, loss);
+  hipLaunchKernelGGL(gradientPartialKernel, dim3((P + 1 + 63) / 64, chunks), dim3(256), 0,
+                     stream(), X, work, m, partial);
+  hipLaunchKernelGGL(gradientFinishKernel, dim3((P + 1 + 255) / 256), dim3(256), 0, stream(),
+                     partial, chunks, m, grad, loss);
   return (int)hipGetLastError();
 }
 

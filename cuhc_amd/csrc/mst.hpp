@@ -23,19 +23,52 @@ struct DevicePoints {
   const float *external = nullptr;
 };
 
+// Edges of a spanning forest in device memory (original indices, u < v).
+struct DeviceEdges {
+  DevArray<int> u, v;
+  DevArray<float> w;
+  int count = 0;
+};
+
+// Device state of Boruvka's contraction (contract.hip).
+struct Contraction {
+  int n = 0;
+  DevArray<unsigned long long> best1;
+  DevArray<unsigned int> best2;
+  DevArray<int> parent, root, flags, newId, changed, counters;
+  DevArray<unsigned char> scanTemp;
+  size_t scanBytes = 0;
+  DeviceEdges edges;
+  int alloc(int points);
+};
+
+// comp[p] = p.
+int resetComponents(int *comp, int n);
+
+// One contraction: every component's smallest outgoing edge under (weight,
+// smaller endpoint, larger endpoint) from nearest[], edges appended to
+// ct.edges, comp relabelled to 0..newNumComp-1. orig maps positions to
+// original indices and posOf back (both null: identity). *added = edges
+// appended this round.
+int contractRound(Contraction &ct, const unsigned long long *nearest, int *comp, const int *orig,
+                  const int *posOf, int numComp, int *newNumComp, int *added);
+
+// Sorts device edges into the global order and writes the scipy-format tree.
+int treeFromDeviceEdges(const DeviceEdges &edges, int n, double *tree);
+
 struct MstOptions {
   bool prune = false;          // points in spatial order; use the walk pass
-  const int *hostOrig = nullptr; // position -> original index (host), null = identity
-  const int *devOrig = nullptr;  // the same on the device
+  const int *devOrig = nullptr;  // position -> original index (device), null = identity
   PassStats *stats = nullptr;
   bool mixed = false;          // FP16 MFMA candidate pass (plain euclidean weights only)
 };
 
 // Exact MST of the complete graph over the points under the total order
-// (weight, smaller original index, larger original index). Edges come back in
-// original indices. Returns 0, a HIP error, or kErrDisconnected.
+// (weight, smaller original index, larger original index). Edges stay in
+// device memory, in original indices. Returns 0, a HIP error, or
+// kErrDisconnected.
 int boruvkaMst(const float *X, int n, int d, MetricSpec m, WeightSpec w, const MstOptions &opt,
-               std::vector<Edge> &edges);
+               Contraction &ct);
 
 // Core distances: core[i] = k-th smallest distance from point i (itself = 1st).
 // kCoreLearned (an extension): radius proposals from the trained network seed
@@ -70,15 +103,16 @@ int singleLinkageMst(const DevicePoints &pts, int n, int d, MetricSpec m, bool p
 // FP16 MFMA certified-candidate pass (mfma.hip): for every point of X (n x d,
 // original order), the exact nearest point in another component under the
 // FP32 euclidean (or squared euclidean) distance, as packKey(weight, j), the
-// same value the FP32 tiles give. hostComp[] are component labels by point;
-// hostPrev (null in the first round) is the previous round's result: a point
-// whose previous nearest is still in another component keeps it (the set of
-// other-component points only shrinks), so only the other rows are searched.
+// same value the FP32 tiles give. comp[] (device) are component labels by
+// point; nearest[] holds the previous round's result unless firstRound: a
+// point whose previous nearest is still in another component keeps it (the
+// set of other-component points only shrinks), so only the other rows are
+// searched.
 struct MixedPass;
 MixedPass *mixedPassCreate(const float *X, int n, int d, int *status);
 void mixedPassDestroy(MixedPass *mp);
-int mixedPassRun(MixedPass *mp, const int *hostComp, int numComponents, bool sqeuclidean,
-                 const unsigned long long *hostPrev, unsigned long long *nearestDevice);
+int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeuclidean,
+                 bool firstRound, unsigned long long *nearest);
 bool mixedPassSupports(int d);
 
 } // namespace cuhc

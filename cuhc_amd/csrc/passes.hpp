@@ -65,12 +65,23 @@ struct PassStats {
 
 // Pruned pass for points in spatial order: same contract as
 // nearestOtherSymmetric, except that keys carry orig[j] (device array), so ties
-// break by original index exactly as in an unpruned run. Returns the share of
-// tile pairs computed in *share. rowTileStride > 1 runs every stride-th row
-// tile only (sampling).
+// break by original index exactly as in an unpruned run. done (nullable) marks
+// rows whose nearest[] already holds their answer (markDoneRows); they are not
+// searched. Returns the share of tile pairs computed in *share.
+// rowTileStride > 1 runs every stride-th row tile only (sampling).
 int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, const int *comp,
-                     const int *orig, unsigned long long *nearest, double *share,
-                     PassStats *stats, int rowTileStride = 1);
+                     const int *orig, const int *done, unsigned long long *nearest,
+                     double *share, PassStats *stats, int rowTileStride = 1);
+
+// Prepares nearest[] for the next Boruvka round from the last round's result
+// and the new components: rows whose nearest is still in another component
+// keep it and get done = 1; the others are reset to empty. posOf maps an
+// original index to its position (null: identity).
+int markDoneRows(unsigned long long *nearest, const int *comp, const int *posOf, int n,
+                 int *done);
+
+// posOf[orig[p]] = p.
+int inverseOrder(const int *orig, int n, int *posOf);
 
 // Share of tiles a pruned pass would compute, from a sample of row tiles
 // (Boruvka's first pass: plain weights, every point its own component).

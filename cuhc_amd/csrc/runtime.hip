@@ -136,6 +136,21 @@ void addUnits(int phase, long long units) {
   gUnits[phase] += units;
 }
 
+void addMs(int phase, double ms, long long launches) {
+  if (!gTiming)
+    return;
+  std::lock_guard<std::mutex> lock(gTimingMutex);
+  gMs[phase] += ms;
+  gLaunches[phase] += launches;
+}
+
+double wallClockKHz() {
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0) != hipSuccess || khz <= 0)
+    khz = 100000;
+  return (double)khz;
+}
+
 void resetTimings() {
   collectTimings();
   std::lock_guard<std::mutex> lock(gTimingMutex);

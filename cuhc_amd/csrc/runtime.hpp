@@ -70,7 +70,10 @@ enum Phase : int {
   kPhaseDense = 5,          // dense Lance-Williams loop
   kPhaseRnn = 6,            // RNN merge/compaction kernels
   kPhaseMlp = 7,            // learned-radius network
-  kNumPhases = 8,
+  kPhaseDenseSelect = 8,    // dense loop, device clock: pair selection (phases A, B)
+  kPhaseDenseUpdate = 9,    // dense loop, device clock: Lance-Williams update (C)
+  kPhaseDenseRescan = 10,   // dense loop, device clock: row rescans (D)
+  kNumPhases = 11,
 };
 
 // Times the enclosed launches of one phase; a timer nested inside another of
@@ -88,6 +91,8 @@ void setTiming(bool on);
 void collectTimings(); // syncs the stream and folds finished events into totals
 void readTimings(double *ms, long long *launches, long long *units, int count);
 void addUnits(int phase, long long units); // algorithmic work units (pairs) per phase
+void addMs(int phase, double ms, long long launches); // time measured on the device clock
+double wallClockKHz();                              // frequency of wall_clock64()
 void resetTimings();
 
 } // namespace cuhc
