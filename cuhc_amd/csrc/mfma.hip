@@ -48,6 +48,7 @@
 #include <atomic>
 #include <cmath>
 #include <numeric>
+#include <type_traits>
 
 namespace cuhc {
 
@@ -155,8 +156,25 @@ __global__ void toHalfKernel(const float *X, int n, int d, int DP, const float *
   }
 }
 
-// Permuted copies of the FP16 points and norms; padding columns get zero
-// coordinates and +inf norms (v = +inf, never a candidate).
+// Position of 16-byte chunk k (of DP / 8) of column c within its column in
+// the permuted copy and in LDS: an XOR swizzle that puts the chunk the MFMA
+// operand loads read in the same step for 8 or 16 consecutive columns in
+// different LDS banks (checked for 32 banks with 8-lane phases and 64 banks
+// with 16-lane phases).
+HD int swizzleChunk(int c, int k, int chunks) {
+  if (chunks == 2)
+    return k ^ (((c >> 2) ^ (c >> 3)) & 1);
+  if (chunks == 4)
+    return k ^ (((c >> 1) ^ (c >> 3)) & 3);
+  return k ^ ((c ^ (c >> 3)) & 7);
+}
+
+// Norm slots per column tile in normp (the tile's 128 norms, then padding, so
+// that each of the four waves stages one 256-byte slice).
+constexpr int kNormSlots = 256;
+
+// Permuted copies of the FP16 points (swizzled chunks) and norms; padding
+// columns get zero coordinates and +inf norms (v = +inf, never a candidate).
 __global__ void permuteHalfKernel(const _Float16 *xh, const float *norm, const int *perm, int n,
                                   int nPad, int DP, _Float16 *xhp, float *normp) {
   const int vecs = DP / 8;
@@ -171,9 +189,12 @@ __global__ void permuteHalfKernel(const _Float16 *xh, const float *norm, const i
       for (int i = 0; i < 8; ++i)
         h[i] = (_Float16)0.0f;
     }
-    *reinterpret_cast<half8 *>(xhp + (size_t)p * DP + 8 * v) = h;
-    if (v == 0)
-      normp[p] = p < n ? norm[perm[p]] : INFINITY;
+    *reinterpret_cast<half8 *>(xhp + (size_t)p * DP + 8 * swizzleChunk(p & 127, v, vecs)) = h;
+    if (v == 0) {
+      const size_t slot = (size_t)(p >> 7) * kNormSlots + (p & 127);
+      normp[slot] = p < n ? norm[perm[p]] : INFINITY;
+      normp[slot + 128] = INFINITY;
+    }
   }
 }
 
@@ -225,20 +246,28 @@ DEV void loadBlock(BlockOps<KB> &o, const _Float16 *sx, const float *sn, int cb,
     o.seed[4 * g + 2] = s4.z;
     o.seed[4 * g + 3] = s4.w;
   }
+  const int c = cb * 32 + l32;
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb)
-    o.a[kb] = *reinterpret_cast<const half8 *>(&sx[(cb * 32 + l32) * DP + kb * 16 + 8 * hi]);
+    o.a[kb] = *reinterpret_cast<const half8 *>(
+        &sx[c * DP + 8 * swizzleChunk(c, kb * 2 + hi, DP / 8)]);
 }
 
-// Per-lane candidate state of one row block (this lane's half of the row's columns).
+// Per-lane candidate state of one row block kept in registers (this lane's
+// half of the row's columns). The rest of the row's state (component range,
+// norm and bounds, row minimum) is read rarely and lives in LDS (RowShared).
 struct RowLane {
-  int slot;      // row slot, or -1
-  int cs, ce;    // the row's component: permuted positions [cs, ce)
-  float N, E, D; // norm, accumulation bound, rounding radius
-  float runMin;  // this lane's running minimum of v
-  float rowMin;  // the row's minimum over both halves at the last tile end
-  float th;      // theta of rowMin
-  int count;     // entries recorded in this lane's list
+  float runMin; // this lane's running minimum of v
+  float th;     // theta of the row minimum at the last tile end
+  int count;    // entries recorded in this lane's list
+};
+
+// Struct-of-arrays per (wave, row block, lane): consecutive lanes, consecutive
+// banks.
+struct RowShared {
+  int cs[4][4][64], ce[4][4][64]; // the row's component: permuted positions [cs, ce)
+  float N[4][4][64], E[4][4][64], D[4][4][64]; // norm, accumulation bound, rounding radius
+  float rowMin[4][4][64];                      // the row minimum over both halves
 };
 
 // v for one row block: C seeded with the column norms, A the column points,
@@ -252,17 +281,20 @@ DEV f32x16 mfmaBlock(const BlockOps<KB> &o, const half8 (&bop)[KB]) {
   return acc;
 }
 
-template <int KB>
-DEV void blockEpilogue(f32x16 acc, RowLane &rl, bool anyMask, int j0, int lane, int block,
-                       const CandArgs &a, int hi) {
+// This lane's minimum over its 16 values of one block (same-component columns
+// masked to +inf when the tile crosses the row's component), folded into its
+// running minimum.
+DEV float blockMin(f32x16 acc, RowLane &rl, bool anyMask, int j0, int lane, int cs, int ce) {
   if (anyMask) {
-    const unsigned span = (unsigned)(rl.ce - rl.cs);
+    // Column of register r: j0 + mOfReg(r, lane) = x + (r / 4) * 8 + r % 4.
+    const unsigned span = (unsigned)(ce - cs);
+    const int x = j0 + (lane >> 5) * 4 - cs;
 #pragma unroll
     for (int r = 0; r < 16; ++r)
-      if ((unsigned)(j0 + mOfReg(r, lane) - rl.cs) < span)
+      if ((unsigned)(x + (r >> 2) * 8 + (r & 3)) < span)
         acc[r] = INFINITY;
   }
-  // Minimum of the lane's 16 values as a tree of three-way minima.
+  // A tree of three-way minima.
   const float m0 = fminf(fminf(acc[0], acc[1]), acc[2]);
   const float m1 = fminf(fminf(acc[3], acc[4]), acc[5]);
   const float m2 = fminf(fminf(acc[6], acc[7]), acc[8]);
@@ -270,47 +302,91 @@ DEV void blockEpilogue(f32x16 acc, RowLane &rl, bool anyMask, int j0, int lane, 
   const float m4 = fminf(fminf(acc[12], acc[13]), acc[14]);
   const float m = fminf(fminf(fminf(m0, m1), m2), fminf(fminf(m3, m4), acc[15]));
   rl.runMin = fminf(rl.runMin, m);
-  // Record this lane's 16 columns of the block when they can hold a candidate
-  // (the lane's theta is never below the row's final one, so nothing needed
-  // is missed). th is -inf for rows outside the range and FLT_MAX before the
-  // first tile end, so one comparison also drops wholly masked blocks.
-  if (m <= rl.th) {
-    if (rl.count < kCandCap)
-      a.cand[((size_t)rl.slot * 2 + hi) * kCandCap + rl.count] =
-          make_uint2((unsigned)block, __float_as_uint(m));
-    ++rl.count;
+  return m;
+}
+
+DEV void recordBlock(RowLane &rl, int slot, float m, int block, const CandArgs &a, int hi) {
+  if (rl.count < kCandCap)
+    a.cand[((size_t)slot * 2 + hi) * kCandCap + rl.count] =
+        make_uint2((unsigned)block, __float_as_uint(m));
+  ++rl.count;
+}
+
+// Records this lane's 16 columns of each of two blocks when they can hold a
+// candidate (the lane's theta is never below the row's final one, so nothing
+// needed is missed). th is -inf for rows outside the range and FLT_MAX before
+// the first tile end, so one comparison also drops wholly masked blocks. One
+// branch per pair of blocks in the common case that neither records.
+DEV void recordPair(RowLane &rl, int slot, float mA, float mB, int blockA, const CandArgs &a,
+                    int hi) {
+  if (fminf(mA, mB) <= rl.th) {
+    if (mA <= rl.th)
+      recordBlock(rl, slot, mA, blockA, a, hi);
+    if (mB <= rl.th)
+      recordBlock(rl, slot, mB, blockA + 2, a, hi);
   }
+}
+
+// s_waitcnt with vmcnt = N and the other counters left unconstrained (gfx9
+// encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+template <int N> DEV void waitVmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// LDS-direct loads (global_load_lds): each lane's 16 (or 4) bytes at g land
+// in LDS at lds + lane * 16 (or * 4); lds is wave-uniform. Written as inline
+// assembly so that the compiler, which cannot tell which LDS bytes these loads
+// write, does not add a full vmcnt(0) wait before every LDS read; the kernel
+// waits for them itself (waitVmcnt, then a barrier). M0 is set in the same
+// statement and holds nothing else in this kernel.
+DEV unsigned ldsOffset(const void *lds) {
+  return __builtin_amdgcn_readfirstlane(
+      (unsigned)(size_t)(__attribute__((address_space(3))) const char *)lds);
+}
+DEV void ldsDirectLoad16(const void *g, const void *lds) {
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+               :
+               : "s"(ldsOffset(lds)), "v"(g)
+               : "memory");
+}
+DEV void ldsDirectLoad4(const void *g, const void *lds) {
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dword %1, off"
+               :
+               : "s"(ldsOffset(lds)), "v"(g)
+               : "memory");
 }
 
 template <int KB>
 __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
   constexpr int DP = 16 * KB;
-  constexpr int VEC_PER_TILE = kTileCols * DP / 8; // half8 vectors per column tile
-  constexpr int VPT = VEC_PER_TILE / 256;          // per thread
-  static_assert(VEC_PER_TILE % 256 == 0, "tile staging");
-  __shared__ __align__(16) _Float16 sx[2][kTileCols * DP];
-  __shared__ __align__(16) float sn[2][kTileCols];
+  constexpr int TILE_BYTES = kTileCols * DP * 2;
+  constexpr int LOADS = TILE_BYTES / (256 * 16); // 16-byte LDS-direct loads per thread per tile
+  static_assert(TILE_BYTES % (256 * 16) == 0, "tile staging");
+  constexpr int STAGES = 3;
+  __shared__ __align__(16) _Float16 sx[STAGES][kTileCols * DP];
+  __shared__ __align__(16) float sn[STAGES][kNormSlots];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int hi = lane >> 5, l32 = lane & 31;
   const int rowBase = blockIdx.x * kBlockRows + wave * kWaveRows;
   const double sqrtKappa = sqrt(a.kappa);
 
-  // Row operands (-2 xh_i) and per-row state.
+  // Row operands (-2 xh_i) and per-row state. Row block rb of this wave is
+  // rows rowBase + rb * 32 + l32 (slot); rows at or past numRows are padding
+  // (zero operands, th = -inf: nothing recorded).
+  __shared__ RowShared rs;
   half8 bop[4][KB];
   RowLane rl[4];
 #pragma unroll
   for (int rb = 0; rb < 4; ++rb) {
     const int r = rowBase + rb * 32 + l32;
     const bool valid = r < a.numRows;
-    int orig = 0;
-    rl[rb].slot = valid ? r : -1;
-    rl[rb].cs = 0;
-    rl[rb].ce = 0;
+    int orig = 0, cs = 0, ce = 0;
     if (valid) {
       const int pos = a.rows[3 * r];
-      rl[rb].cs = a.rows[3 * r + 1];
-      rl[rb].ce = a.rows[3 * r + 2];
+      cs = a.rows[3 * r + 1];
+      ce = a.rows[3 * r + 2];
       orig = a.perm[pos];
     }
 #pragma unroll
@@ -325,109 +401,133 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
       }
       bop[rb][kb] = h;
     }
-    rl[rb].N = valid ? a.norm[orig] : 0.0f;
+    rs.cs[wave][rb][lane] = cs;
+    rs.ce[wave][rb][lane] = ce;
+    rs.N[wave][rb][lane] = valid ? a.norm[orig] : 0.0f;
     const float Rs = (valid ? a.R[orig] : 0.0f) + a.Rmax;
-    rl[rb].E = a.gamma * Rs * Rs * (1.0f + 1e-6f);
-    rl[rb].D = (valid ? a.rho[orig] : 0.0f) + a.rhoMax;
+    rs.E[wave][rb][lane] = a.gamma * Rs * Rs * (1.0f + 1e-6f);
+    rs.D[wave][rb][lane] = (valid ? a.rho[orig] : 0.0f) + a.rhoMax;
+    rs.rowMin[wave][rb][lane] = INFINITY;
     rl[rb].runMin = INFINITY;
-    rl[rb].rowMin = INFINITY;
     rl[rb].th = valid ? FLT_MAX : -INFINITY;
     rl[rb].count = 0;
   }
 
-  // Column tiles stream through LDS, double-buffered: the global loads of
-  // tile t + 1 are issued before tile t is computed.
-  const half8 *gsrc = reinterpret_cast<const half8 *>(a.xhp);
-  half8 pf[VPT];
-  float pfn = 0.0f;
-  auto fetch = [&](int t) {
+  // Column tiles stream into LDS by direct (LDS-DMA) loads, three buffers
+  // deep: tile t + 2 is requested while tile t is computed. Each wave copies
+  // LOADS consecutive 1 KB slices of the tile and one 256-byte slice of its
+  // norm slots; the slices land in LDS in global order, and the global copy is
+  // already swizzled for the operand loads.
+  const char *gsrc = reinterpret_cast<const char *>(a.xhp);
+  const char *gnorm = reinterpret_cast<const char *>(a.normp);
+  auto request = [&](int t) {
+    char *dstTile = reinterpret_cast<char *>(sx[t % STAGES]);
 #pragma unroll
-    for (int v = 0; v < VPT; ++v)
-      pf[v] = gsrc[(size_t)t * VEC_PER_TILE + threadIdx.x + v * 256];
-    if (threadIdx.x < kTileCols)
-      pfn = a.normp[(size_t)t * kTileCols + threadIdx.x];
+    for (int j = 0; j < LOADS; ++j) {
+      const int slice = j * 4 + wave;
+      ldsDirectLoad16(gsrc + (size_t)t * TILE_BYTES + slice * 1024 + lane * 16,
+                      dstTile + slice * 1024);
+    }
+    ldsDirectLoad4(gnorm + ((size_t)t * kNormSlots + wave * 64 + lane) * 4,
+                   reinterpret_cast<char *>(sn[t % STAGES]) + wave * 256);
   };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int v = 0; v < VPT; ++v)
-      reinterpret_cast<half8 *>(sx[buf])[threadIdx.x + v * 256] = pf[v];
-    if (threadIdx.x < kTileCols)
-      sn[buf][threadIdx.x] = pfn;
-  };
-
   const int numTiles = a.nPad / kTileCols;
   const int validRows = max(0, min(kWaveRows, a.numRows - rowBase));
   unsigned long long computedTiles = 0;
-  fetch(0);
-  store(0);
-  __syncthreads();
+  request(0);
+  if (numTiles > 1)
+    request(1);
   for (int t = 0; t < numTiles; ++t) {
-    const int buf = t & 1;
+    const int buf = t % STAGES;
+    // Tile t's loads are this wave's oldest outstanding ones; only tile t + 1's
+    // (LOADS + 1) may remain (memory operations complete in order).
     if (t + 1 < numTiles)
-      fetch(t + 1);
+      waitVmcnt<LOADS + 1>();
+    else
+      waitVmcnt<0>();
+    // Every wave's slices of tile t are in LDS, and every wave is done with
+    // tile t - 1, whose buffer tile t + 2 now reuses.
+    __syncthreads();
+    if (t + 2 < numTiles)
+      request(t + 2);
     const int c0 = t * kTileCols, c1 = c0 + kTileCols;
     // Wave-uniform decisions about the same-component mask.
     bool inter = false, contained = true;
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) {
-      if (rl[rb].slot < 0)
+      if (rowBase + rb * 32 + l32 >= a.numRows)
         continue;
-      inter |= rl[rb].ce > c0 && rl[rb].cs < c1;
-      contained &= rl[rb].cs <= c0 && rl[rb].ce >= c1;
+      const int cs = rs.cs[wave][rb][lane], ce = rs.ce[wave][rb][lane];
+      inter |= ce > c0 && cs < c1;
+      contained &= cs <= c0 && ce >= c1;
     }
     const bool skipAll = __ballot(!contained) == 0;
     const bool anyMask = __ballot(inter) != 0;
     if (!skipAll) {
       ++computedTiles;
-      BlockOps<KB> o0, o1;
-      loadBlock<KB>(o0, sx[buf], sn[buf], 0, l32, hi);
+      // The masked and the mask-free tile are separate code paths, so the
+      // inner loop carries no per-block mask branch.
+      auto runTile = [&](auto maskTag) {
+        constexpr bool kMask = decltype(maskTag)::value;
+        BlockOps<KB> o0, o1;
+        loadBlock<KB>(o0, sx[buf], sn[buf], 0, l32, hi);
 #pragma unroll 1
-      for (int cb = 0; cb < kTileCols / 32; cb += 2) {
-        // All four row blocks' MFMA chains are issued before any epilogue
-        // reads a result, so they overlap in the matrix pipeline.
-        f32x16 acc[4];
-        loadBlock<KB>(o1, sx[buf], sn[buf], cb + 1, l32, hi);
+        for (int cb = 0; cb < kTileCols / 32; cb += 2) {
+          // All four row blocks' MFMA chains are issued before any epilogue
+          // reads a result, so they overlap in the matrix pipeline.
+          f32x16 acc[4];
+          loadBlock<KB>(o1, sx[buf], sn[buf], cb + 1, l32, hi);
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-          acc[rb] = mfmaBlock<KB>(o0, bop[rb]);
+          for (int rb = 0; rb < 4; ++rb)
+            acc[rb] = mfmaBlock<KB>(o0, bop[rb]);
+          float mA[4];
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-          blockEpilogue<KB>(acc[rb], rl[rb], anyMask, c0 + cb * 32, lane, t * 8 + cb * 2 + hi, a,
-                            hi);
-        if (cb + 2 < kTileCols / 32)
-          loadBlock<KB>(o0, sx[buf], sn[buf], cb + 2, l32, hi);
+          for (int rb = 0; rb < 4; ++rb)
+            mA[rb] = blockMin(acc[rb], rl[rb], kMask, c0 + cb * 32, lane,
+                              kMask ? rs.cs[wave][rb][lane] : 0,
+                              kMask ? rs.ce[wave][rb][lane] : 0);
+          if (cb + 2 < kTileCols / 32)
+            loadBlock<KB>(o0, sx[buf], sn[buf], cb + 2, l32, hi);
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-          acc[rb] = mfmaBlock<KB>(o1, bop[rb]);
+          for (int rb = 0; rb < 4; ++rb)
+            acc[rb] = mfmaBlock<KB>(o1, bop[rb]);
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-          blockEpilogue<KB>(acc[rb], rl[rb], anyMask, c0 + (cb + 1) * 32, lane,
-                            t * 8 + (cb + 1) * 2 + hi, a, hi);
-      }
+          for (int rb = 0; rb < 4; ++rb) {
+            const float mB = blockMin(acc[rb], rl[rb], kMask, c0 + (cb + 1) * 32, lane,
+                                      kMask ? rs.cs[wave][rb][lane] : 0,
+                                      kMask ? rs.ce[wave][rb][lane] : 0);
+            recordPair(rl[rb], rowBase + rb * 32 + l32, mA[rb], mB, t * 8 + cb * 2 + hi, a, hi);
+          }
+        }
+      };
+      if (anyMask)
+        runTile(std::true_type{});
+      else
+        runTile(std::false_type{});
       // Tile end: both halves of each row agree on the row minimum and theta.
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) {
         const float m = fminf(rl[rb].runMin, __shfl_xor(rl[rb].runMin, 32, 64));
         rl[rb].runMin = m;
-        if (m < rl[rb].rowMin && rl[rb].slot >= 0) {
-          rl[rb].rowMin = m;
-          rl[rb].th = thetaOf(m, rl[rb].N, rl[rb].E, rl[rb].D, sqrtKappa);
+        if (m < rs.rowMin[wave][rb][lane] && rowBase + rb * 32 + l32 < a.numRows) {
+          rs.rowMin[wave][rb][lane] = m;
+          rl[rb].th = thetaOf(m, rs.N[wave][rb][lane], rs.E[wave][rb][lane],
+                              rs.D[wave][rb][lane], sqrtKappa);
         }
       }
     }
-    if (t + 1 < numTiles)
-      store(buf ^ 1);
-    __syncthreads();
   }
   unsigned long long rec = 0;
 #pragma unroll
-  for (int rb = 0; rb < 4; ++rb)
-    if (rl[rb].slot >= 0) {
-      a.candCount[(size_t)rl[rb].slot * 2 + hi] = rl[rb].count;
+  for (int rb = 0; rb < 4; ++rb) {
+    const int slot = rowBase + rb * 32 + l32;
+    if (slot < a.numRows) {
+      a.candCount[(size_t)slot * 2 + hi] = rl[rb].count;
       if (hi == 0)
-        a.theta[rl[rb].slot] = rl[rb].th;
+        a.theta[slot] = rl[rb].th;
       rec += rl[rb].count;
     }
+  }
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1)
     rec += __shfl_xor(rec, m, 64);
@@ -668,7 +768,7 @@ MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
   MixedPass *mp = new MixedPass;
   mp->n = n;
   mp->d = d;
-  mp->DP = (d + 15) / 16 * 16;
+  mp->DP = d <= 16 ? 16 : d <= 32 ? 32 : 64; // a power of two: swizzled 16-byte chunks
   mp->nPad = (n + kTileCols - 1) / kTileCols * kTileCols;
   mp->X = X;
   Status s;
@@ -680,7 +780,7 @@ MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
   s(mp->xh.alloc((size_t)n * mp->DP));
   s(mp->xhp.alloc((size_t)mp->nPad * mp->DP));
   s(mp->norm.alloc(n));
-  s(mp->normp.alloc(mp->nPad));
+  s(mp->normp.alloc((size_t)mp->nPad / kTileCols * kNormSlots));
   s(mp->R.alloc(n));
   s(mp->rho.alloc(n));
   s(mp->perm.alloc(n));
@@ -828,7 +928,6 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
     switch (KB) {
     case 1: hipLaunchKernelGGL(mfmaCandidateKernel<1>, dim3(blocks), dim3(256), 0, stream(), ca); break;
     case 2: hipLaunchKernelGGL(mfmaCandidateKernel<2>, dim3(blocks), dim3(256), 0, stream(), ca); break;
-    case 3: hipLaunchKernelGGL(mfmaCandidateKernel<3>, dim3(blocks), dim3(256), 0, stream(), ca); break;
     default: hipLaunchKernelGGL(mfmaCandidateKernel<4>, dim3(blocks), dim3(256), 0, stream(), ca); break;
     }
     s((int)hipGetLastError());
