@@ -207,7 +207,7 @@ struct CandArgs {
   const _Float16 *xh;  // n x DP, original order (row operands)
   const float *norm, *R, *rho; // per point, original order
   const int *perm;     // position -> original index
-  const int *rows;     // numRows x 3: position, component start, component end
+  const int *rows;     // numRows x 3: position (-1: padding slot), component start, end
   int numRows, n, nPad;
   float Rmax, rhoMax;
   float gamma;  // (K + 8) 2^-23
@@ -373,22 +373,25 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
   const double sqrtKappa = sqrt(a.kappa);
 
   // Row operands (-2 xh_i) and per-row state. Row block rb of this wave is
-  // rows rowBase + rb * 32 + l32 (slot); rows at or past numRows are padding
-  // (zero operands, th = -inf: nothing recorded).
+  // rows rowBase + rb * 32 + l32 (slot); slots at or past numRows and slots
+  // with position -1 are padding (zero operands, th = -inf: nothing recorded,
+  // component start -1 in LDS).
   __shared__ RowShared rs;
+  int myValid = 0; // valid rows among this lane's (hi == 0 lanes count each row once)
   half8 bop[4][KB];
   RowLane rl[4];
 #pragma unroll
   for (int rb = 0; rb < 4; ++rb) {
     const int r = rowBase + rb * 32 + l32;
-    const bool valid = r < a.numRows;
-    int orig = 0, cs = 0, ce = 0;
+    const int pos = r < a.numRows ? a.rows[3 * r] : -1;
+    const bool valid = pos >= 0;
+    int orig = 0, cs = -1, ce = 0;
     if (valid) {
-      const int pos = a.rows[3 * r];
       cs = a.rows[3 * r + 1];
       ce = a.rows[3 * r + 2];
       orig = a.perm[pos];
     }
+    myValid += valid && hi == 0;
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) {
       half8 h;
@@ -420,8 +423,8 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
   // already swizzled for the operand loads.
   const char *gsrc = reinterpret_cast<const char *>(a.xhp);
   const char *gnorm = reinterpret_cast<const char *>(a.normp);
-  auto request = [&](int t) {
-    char *dstTile = reinterpret_cast<char *>(sx[t % STAGES]);
+  auto request = [&](int t, int stage) {
+    char *dstTile = reinterpret_cast<char *>(sx[stage]);
 #pragma unroll
     for (int j = 0; j < LOADS; ++j) {
       const int slice = j * 4 + wave;
@@ -429,35 +432,58 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
                       dstTile + slice * 1024);
     }
     ldsDirectLoad4(gnorm + ((size_t)t * kNormSlots + wave * 64 + lane) * 4,
-                   reinterpret_cast<char *>(sn[t % STAGES]) + wave * 256);
+                   reinterpret_cast<char *>(sn[stage]) + wave * 256);
   };
   const int numTiles = a.nPad / kTileCols;
-  const int validRows = max(0, min(kWaveRows, a.numRows - rowBase));
+  int validRows = myValid;
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1)
+    validRows += __shfl_xor(validRows, m, 64);
   unsigned long long computedTiles = 0;
-  request(0);
-  if (numTiles > 1)
-    request(1);
-  for (int t = 0; t < numTiles; ++t) {
-    const int buf = t % STAGES;
-    // Tile t's loads are this wave's oldest outstanding ones; only tile t + 1's
-    // (LOADS + 1) may remain (memory operations complete in order).
-    if (t + 1 < numTiles)
+  // Column tiles [tA, tB) lie inside the component of every row of the
+  // workgroup (rows are listed in component order, so late rounds with large
+  // components skip most tiles): they are neither loaded nor visited.
+  __shared__ int blockMaxCs, blockMinCe;
+  if (threadIdx.x == 0) {
+    blockMaxCs = 0;
+    blockMinCe = 0x7fffffff;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+    if (rs.cs[wave][rb][lane] >= 0) {
+      atomicMax(&blockMaxCs, rs.cs[wave][rb][lane]);
+      atomicMin(&blockMinCe, rs.ce[wave][rb][lane]);
+    }
+  __syncthreads();
+  const int tA = (blockMaxCs + kTileCols - 1) / kTileCols, tB = blockMinCe / kTileCols;
+  const int skipped = tB > tA ? tB - tA : 0;
+  const int iters = numTiles - skipped;
+  auto tileOf = [&](int i) { return skipped && i >= tA ? i + skipped : i; };
+  request(tileOf(0), 0);
+  if (iters > 1)
+    request(tileOf(1), 1);
+  for (int i = 0; i < iters; ++i) {
+    const int t = tileOf(i), buf = i % STAGES;
+    // Tile t's loads are this wave's oldest outstanding ones; only the next
+    // tile's (LOADS + 1) may remain (memory operations complete in order).
+    if (i + 1 < iters)
       waitVmcnt<LOADS + 1>();
     else
       waitVmcnt<0>();
     // Every wave's slices of tile t are in LDS, and every wave is done with
-    // tile t - 1, whose buffer tile t + 2 now reuses.
+    // the previous tile, whose buffer the tile after next now reuses.
     __syncthreads();
-    if (t + 2 < numTiles)
-      request(t + 2);
+    if (i + 2 < iters)
+      request(tileOf(i + 2), (i + 2) % STAGES);
     const int c0 = t * kTileCols, c1 = c0 + kTileCols;
     // Wave-uniform decisions about the same-component mask.
     bool inter = false, contained = true;
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) {
-      if (rowBase + rb * 32 + l32 >= a.numRows)
-        continue;
       const int cs = rs.cs[wave][rb][lane], ce = rs.ce[wave][rb][lane];
+      if (cs < 0)
+        continue; // padding slot
       inter |= ce > c0 && cs < c1;
       contained &= cs <= c0 && ce >= c1;
     }
@@ -509,7 +535,7 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
       for (int rb = 0; rb < 4; ++rb) {
         const float m = fminf(rl[rb].runMin, __shfl_xor(rl[rb].runMin, 32, 64));
         rl[rb].runMin = m;
-        if (m < rs.rowMin[wave][rb][lane] && rowBase + rb * 32 + l32 < a.numRows) {
+        if (m < rs.rowMin[wave][rb][lane] && rs.cs[wave][rb][lane] >= 0) {
           rs.rowMin[wave][rb][lane] = m;
           rl[rb].th = thetaOf(m, rs.N[wave][rb][lane], rs.E[wave][rb][lane],
                               rs.D[wave][rb][lane], sqrtKappa);
@@ -521,7 +547,7 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
 #pragma unroll
   for (int rb = 0; rb < 4; ++rb) {
     const int slot = rowBase + rb * 32 + l32;
-    if (slot < a.numRows) {
+    if (rs.cs[wave][rb][lane] >= 0) {
       a.candCount[(size_t)slot * 2 + hi] = rl[rb].count;
       if (hi == 0)
         a.theta[slot] = rl[rb].th;
@@ -609,8 +635,8 @@ __global__ void __launch_bounds__(256) recheckKernel(RecheckArgs a) {
   __shared__ float rowX[4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = blockIdx.x * 4 + wave;
-  if (r >= a.numRows)
-    return; // wave-uniform; no block-wide barrier below
+  if (r >= a.numRows || a.rows[3 * r] < 0)
+    return; // wave-uniform (padding slot or past the end); no block-wide barrier below
   // Independent loads: the row, its counts, theta and every list slot (slots
   // past a count hold stale data and are masked below).
   const int pos = a.rows[3 * r], cs = a.rows[3 * r + 1], ce = a.rows[3 * r + 2];
@@ -709,15 +735,38 @@ __global__ void staleRowsKernel(const int *__restrict__ perm, const int *__restr
   flag[i] = stale ? 1 : 0;
 }
 
+// Row slots: a component with at least a workgroup's worth of stale rows gets
+// whole workgroups of its own (its row count rounded up, padding slots marked
+// by position -1), so that its workgroups can skip its own column range; the
+// other components' rows are packed after them.
+__global__ void componentSlotsKernel(const int *__restrict__ start, const int *__restrict__ end,
+                                     const int *__restrict__ index, int numComp, int *bigSlots,
+                                     int *smallSlots) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > numComp)
+    return;
+  if (c == numComp) {
+    bigSlots[c] = 0;
+    smallSlots[c] = 0;
+    return;
+  }
+  const int count = index[end[c]] - index[start[c]];
+  const bool big = count >= kBlockRows;
+  bigSlots[c] = big ? (count + kBlockRows - 1) / kBlockRows * kBlockRows : 0;
+  smallSlots[c] = big ? 0 : count;
+}
+
 __global__ void rowListKernel(const int *__restrict__ perm, const int *__restrict__ comp,
                               const int *__restrict__ flag, const int *__restrict__ index,
-                              const int *__restrict__ start, const int *__restrict__ end, int n,
-                              int *rows) {
+                              const int *__restrict__ start, const int *__restrict__ end,
+                              const int *__restrict__ bigSlots, const int *__restrict__ bigBase,
+                              const int *__restrict__ smallBase, int numComp, int n, int *rows) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || !flag[i])
     return;
   const int c = comp[perm[i]];
-  const int r = index[i];
+  const int within = index[i] - index[start[c]];
+  const int r = bigSlots[c] ? bigBase[c] + within : bigBase[numComp] + smallBase[c] + within;
   rows[3 * r] = i;
   rows[3 * r + 1] = start[c];
   rows[3 * r + 2] = end[c];
@@ -754,6 +803,7 @@ struct MixedPass {
   DevArray<float> norm, normp, R, rho, mu, theta, xp;
   DevArray<int> perm, rows, candCount;
   DevArray<int> positions, sortedComp, compStart, compEnd, rowFlag, rowIndex;
+  DevArray<int> bigSlots, smallSlots, bigBase, smallBase;
   DevArray<unsigned char> sortTemp, scanTemp;
   size_t sortBytes = 0, scanBytes = 0;
   DevArray<uint2> cand;
@@ -784,11 +834,12 @@ MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
   s(mp->R.alloc(n));
   s(mp->rho.alloc(n));
   s(mp->perm.alloc(n));
-  s(mp->rows.alloc((size_t)3 * n));
-  s(mp->candCount.alloc((size_t)2 * n));
-  s(mp->theta.alloc(n));
+  // Row slots: at most twice the rows (workgroup-aligned components, padding).
+  s(mp->rows.alloc((size_t)3 * 2 * n));
+  s(mp->candCount.alloc((size_t)2 * 2 * n));
+  s(mp->theta.alloc((size_t)2 * n));
   s(mp->xp.alloc((size_t)n * d));
-  s(mp->cand.alloc((size_t)2 * n * kCandCap));
+  s(mp->cand.alloc((size_t)2 * 2 * n * kCandCap));
   s(mp->stats.alloc(5));
   s(mp->recheckStats.alloc((size_t)kStatStripes * kStripeWords));
   s(mp->positions.alloc(n));
@@ -797,6 +848,10 @@ MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
   s(mp->compEnd.alloc(n));
   s(mp->rowFlag.alloc(n + 1));
   s(mp->rowIndex.alloc(n + 1));
+  s(mp->bigSlots.alloc(n + 1));
+  s(mp->smallSlots.alloc(n + 1));
+  s(mp->bigBase.alloc(n + 1));
+  s(mp->smallBase.alloc(n + 1));
   s((int)hipcub::DeviceRadixSort::SortPairs(nullptr, mp->sortBytes, (unsigned int *)nullptr,
                                             (unsigned int *)nullptr, (int *)nullptr,
                                             (int *)nullptr, n, 0, 32, stream()));
@@ -888,12 +943,25 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   bytes = mp->scanBytes;
   s((int)hipcub::DeviceScan::ExclusiveSum(mp->scanTemp.get(), bytes, mp->rowFlag.get(),
                                           mp->rowIndex.get(), n + 1, stream()));
+  hipLaunchKernelGGL(componentSlotsKernel, dim3((numComponents + 1 + 255) / 256), dim3(256), 0,
+                     stream(), mp->compStart.get(), mp->compEnd.get(), mp->rowIndex.get(),
+                     numComponents, mp->bigSlots.get(), mp->smallSlots.get());
+  bytes = mp->scanBytes;
+  s((int)hipcub::DeviceScan::ExclusiveSum(mp->scanTemp.get(), bytes, mp->bigSlots.get(),
+                                          mp->bigBase.get(), numComponents + 1, stream()));
+  bytes = mp->scanBytes;
+  s((int)hipcub::DeviceScan::ExclusiveSum(mp->scanTemp.get(), bytes, mp->smallSlots.get(),
+                                          mp->smallBase.get(), numComponents + 1, stream()));
+  s(fill(mp->rows, 0xff, mp->rows.bytes())); // every slot starts as padding (position -1)
   hipLaunchKernelGGL(rowListKernel, grid, dim3(256), 0, stream(), mp->perm.get(), comp,
                      mp->rowFlag.get(), mp->rowIndex.get(), mp->compStart.get(),
-                     mp->compEnd.get(), n, mp->rows.get());
+                     mp->compEnd.get(), mp->bigSlots.get(), mp->bigBase.get(),
+                     mp->smallBase.get(), numComponents, n, mp->rows.get());
   s((int)hipGetLastError());
-  int numRows = 0;
-  s(download(&numRows, mp->rowIndex.get() + n, sizeof(int)));
+  int slotTotals[2] = {0, 0};
+  s(download(&slotTotals[0], mp->bigBase.get() + numComponents, sizeof(int)));
+  s(download(&slotTotals[1], mp->smallBase.get() + numComponents, sizeof(int)));
+  const int numRows = slotTotals[0] + slotTotals[1]; // row slots, padding included
   if (!s.ok() || numRows == 0)
     return s.code;
   hipLaunchKernelGGL(permuteHalfKernel, dim3(2048), dim3(256), 0, stream(), mp->xh.get(),
