@@ -213,10 +213,22 @@ struct CandArgs {
   float gamma;  // (K + 8) 2^-23
   double kappa; // ((1 + eps)(1 + 2^-21) / (1 - eps)) with eps = (d + 4) 2^-24
   uint2 *cand;  // (numRows x 2 halves) x kCandCap: (half-block, block minimum bits)
-  int *candCount; // numRows x 2
-  float *theta;   // numRows
+  int *candCount; // numRows x 2 (zeroed before a launch with column splits)
+  unsigned int *theta;  // numRows, order-preserving bits (thetaBits), minimum over splits
+  unsigned int *rowMin; // numRows, thetaBits of the row minimum shared by column splits
   unsigned long long *recorded; // [0] entries recorded, [4] row-column pairs computed
 };
+
+// Order-preserving 32-bit image of a float (any sign), for atomicMin across
+// the workgroups that share a row; 0xffffffff is "none".
+HD unsigned int thetaBits(float f) {
+  const unsigned int b = bitsOf(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+HD float thetaFromBits(unsigned int o) {
+  return floatOf((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+constexpr unsigned int kNoTheta = 0xffffffffu;
 
 __device__ __noinline__ float thetaOf(float m, float Ni, float Ei, float deltai, double sqrtKappa) {
   if (!(m < INFINITY))
@@ -305,10 +317,15 @@ DEV float blockMin(f32x16 acc, RowLane &rl, bool anyMask, int j0, int lane, int 
   return m;
 }
 
-DEV void recordBlock(RowLane &rl, int slot, float m, int block, const CandArgs &a, int hi) {
-  if (rl.count < kCandCap)
-    a.cand[((size_t)slot * 2 + hi) * kCandCap + rl.count] =
-        make_uint2((unsigned)block, __float_as_uint(m));
+// One list per (row, half). With column splits several workgroups append to
+// it, so the position comes from an atomic counter; otherwise from the lane's
+// own count (stored at the end).
+DEV void recordBlock(RowLane &rl, int slot, float m, int block, const CandArgs &a, int hi,
+                     bool shared) {
+  const size_t list = (size_t)slot * 2 + hi;
+  const int pos = shared ? atomicAdd(&a.candCount[list], 1) : rl.count;
+  if (pos < kCandCap)
+    a.cand[list * kCandCap + pos] = make_uint2((unsigned)block, __float_as_uint(m));
   ++rl.count;
 }
 
@@ -318,12 +335,12 @@ DEV void recordBlock(RowLane &rl, int slot, float m, int block, const CandArgs &
 // the first tile end, so one comparison also drops wholly masked blocks. One
 // branch per pair of blocks in the common case that neither records.
 DEV void recordPair(RowLane &rl, int slot, float mA, float mB, int blockA, const CandArgs &a,
-                    int hi) {
+                    int hi, bool shared) {
   if (fminf(mA, mB) <= rl.th) {
     if (mA <= rl.th)
-      recordBlock(rl, slot, mA, blockA, a, hi);
+      recordBlock(rl, slot, mA, blockA, a, hi, shared);
     if (mB <= rl.th)
-      recordBlock(rl, slot, mB, blockA + 2, a, hi);
+      recordBlock(rl, slot, mB, blockA + 2, a, hi, shared);
   }
 }
 
@@ -458,23 +475,39 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
   __syncthreads();
   const int tA = (blockMaxCs + kTileCols - 1) / kTileCols, tB = blockMinCe / kTileCols;
   const int skipped = tB > tA ? tB - tA : 0;
-  const int iters = numTiles - skipped;
-  auto tileOf = [&](int i) { return skipped && i >= tA ? i + skipped : i; };
-  request(tileOf(0), 0);
-  if (iters > 1)
-    request(tileOf(1), 1);
-  for (int i = 0; i < iters; ++i) {
-    const int t = tileOf(i), buf = i % STAGES;
+  const int allIters = numTiles - skipped;
+  // Column split blockIdx.y of gridDim.y takes an equal share of the tiles
+  // (several workgroups per row block when rows are few).
+  const int splits = gridDim.y;
+  const bool shared = splits > 1;
+  const int it0 = (int)((long long)allIters * blockIdx.y / splits);
+  const int iters = (int)((long long)allIters * (blockIdx.y + 1) / splits) - it0;
+  auto tileOf = [&](int i) {
+    const int j = it0 + i;
+    return skipped && j >= tA ? j + skipped : j;
+  };
+  if (iters > 0) {
+    request(tileOf(0), 0);
+    if (iters > 1)
+      request(tileOf(1), 1);
+  }
+  // Iteration -1 is a dry run over the first tile: it sets every row's theta
+  // from that tile's minimum without recording anything, so the first tile's
+  // blocks are recorded against a real theta instead of an unbounded one.
+  for (int i = iters > 0 ? -1 : 0; i < iters; ++i) {
+    const bool dry = i < 0;
+    const int ii = dry ? 0 : i;
+    const int t = tileOf(ii), buf = ii % STAGES;
     // Tile t's loads are this wave's oldest outstanding ones; only the next
     // tile's (LOADS + 1) may remain (memory operations complete in order).
-    if (i + 1 < iters)
+    if (ii + 1 < iters)
       waitVmcnt<LOADS + 1>();
     else
       waitVmcnt<0>();
     // Every wave's slices of tile t are in LDS, and every wave is done with
     // the previous tile, whose buffer the tile after next now reuses.
     __syncthreads();
-    if (i + 2 < iters)
+    if (!dry && i + 2 < iters)
       request(tileOf(i + 2), (i + 2) % STAGES);
     const int c0 = t * kTileCols, c1 = c0 + kTileCols;
     // Wave-uniform decisions about the same-component mask.
@@ -490,7 +523,7 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
     const bool skipAll = __ballot(!contained) == 0;
     const bool anyMask = __ballot(inter) != 0;
     if (!skipAll) {
-      ++computedTiles;
+      computedTiles += !dry;
       // The masked and the mask-free tile are separate code paths, so the
       // inner loop carries no per-block mask branch.
       auto runTile = [&](auto maskTag) {
@@ -522,7 +555,9 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
             const float mB = blockMin(acc[rb], rl[rb], kMask, c0 + (cb + 1) * 32, lane,
                                       kMask ? rs.cs[wave][rb][lane] : 0,
                                       kMask ? rs.ce[wave][rb][lane] : 0);
-            recordPair(rl[rb], rowBase + rb * 32 + l32, mA[rb], mB, t * 8 + cb * 2 + hi, a, hi);
+            if (!dry)
+              recordPair(rl[rb], rowBase + rb * 32 + l32, mA[rb], mB, t * 8 + cb * 2 + hi, a,
+                         hi, shared);
           }
         }
       };
@@ -533,9 +568,19 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
       // Tile end: both halves of each row agree on the row minimum and theta.
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) {
-        const float m = fminf(rl[rb].runMin, __shfl_xor(rl[rb].runMin, 32, 64));
+        float m = fminf(rl[rb].runMin, __shfl_xor(rl[rb].runMin, 32, 64));
         rl[rb].runMin = m;
-        if (m < rs.rowMin[wave][rb][lane] && rs.cs[wave][rb][lane] >= 0) {
+        const bool improved = m < rs.rowMin[wave][rb][lane] && rs.cs[wave][rb][lane] >= 0;
+        if (shared) {
+          // Publish an improvement; take the best any split has published.
+          unsigned int old = kNoTheta;
+          if (improved && hi == 0)
+            old = atomicMin(&a.rowMin[rowBase + rb * 32 + l32], thetaBits(m));
+          old = __shfl(old, l32, 64);
+          if (old != kNoTheta)
+            m = fminf(m, thetaFromBits(old));
+        }
+        if (improved) {
           rs.rowMin[wave][rb][lane] = m;
           rl[rb].th = thetaOf(m, rs.N[wave][rb][lane], rs.E[wave][rb][lane],
                               rs.D[wave][rb][lane], sqrtKappa);
@@ -548,9 +593,10 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
   for (int rb = 0; rb < 4; ++rb) {
     const int slot = rowBase + rb * 32 + l32;
     if (rs.cs[wave][rb][lane] >= 0) {
-      a.candCount[(size_t)slot * 2 + hi] = rl[rb].count;
+      if (!shared)
+        a.candCount[(size_t)slot * 2 + hi] = rl[rb].count;
       if (hi == 0)
-        a.theta[slot] = rl[rb].th;
+        atomicMin(&a.theta[slot], thetaBits(rl[rb].th));
       rec += rl[rb].count;
     }
   }
@@ -579,7 +625,7 @@ struct RecheckArgs {
   int numRows;
   const uint2 *cand;
   const int *candCount;
-  const float *theta;
+  const unsigned int *theta; // thetaBits
   bool sqeuclidean;
   unsigned long long *nearest; // by original index
   // kStatStripes stripes of one 128-byte line each: [1] half-blocks rechecked,
@@ -641,7 +687,7 @@ __global__ void __launch_bounds__(256) recheckKernel(RecheckArgs a) {
   // past a count hold stale data and are masked below).
   const int pos = a.rows[3 * r], cs = a.rows[3 * r + 1], ce = a.rows[3 * r + 2];
   const int c0 = a.candCount[(size_t)r * 2], c1 = a.candCount[(size_t)r * 2 + 1];
-  const float th = a.theta[r];
+  const float th = thetaFromBits(a.theta[r]);
   const uint2 entry = a.cand[(size_t)r * 2 * kCandCap + lane];
   float *xi = rowX[wave];
   for (int k = lane; k < a.d; k += 64)
@@ -800,7 +846,8 @@ struct MixedPass {
   int n = 0, d = 0, DP = 0, nPad = 0;
   const float *X = nullptr;
   DevArray<_Float16> xh, xhp;
-  DevArray<float> norm, normp, R, rho, mu, theta, xp;
+  DevArray<float> norm, normp, R, rho, mu, xp;
+  DevArray<unsigned int> theta, rowMin;
   DevArray<int> perm, rows, candCount;
   DevArray<int> positions, sortedComp, compStart, compEnd, rowFlag, rowIndex;
   DevArray<int> bigSlots, smallSlots, bigBase, smallBase;
@@ -838,6 +885,7 @@ MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
   s(mp->rows.alloc((size_t)3 * 2 * n));
   s(mp->candCount.alloc((size_t)2 * 2 * n));
   s(mp->theta.alloc((size_t)2 * n));
+  s(mp->rowMin.alloc((size_t)2 * n));
   s(mp->xp.alloc((size_t)n * d));
   s(mp->cand.alloc((size_t)2 * 2 * n * kCandCap));
   s(mp->stats.alloc(5));
@@ -989,14 +1037,25 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   ca.cand = mp->cand;
   ca.candCount = mp->candCount;
   ca.theta = mp->theta;
+  ca.rowMin = mp->rowMin;
   ca.recorded = mp->stats.get();
   const int blocks = (numRows + kBlockRows - 1) / kBlockRows;
+  // Few rows (late rounds): split the columns so that every CU gets two
+  // workgroups; each split keeps at least 16 tiles.
+  const int target = 2 * numCUs();
+  int splits = blocks >= target ? 1 : (target + blocks - 1) / blocks;
+  splits = std::max(1, std::min({splits, 8, mp->nPad / kTileCols / 16}));
+  s(fill(mp->theta, 0xff, (size_t)numRows * sizeof(unsigned int)));
+  if (splits > 1) {
+    s(fill(mp->candCount, 0, (size_t)numRows * 2 * sizeof(int)));
+    s(fill(mp->rowMin, 0xff, (size_t)numRows * sizeof(unsigned int)));
+  }
   {
     PhaseTimer timer(kPhaseMfmaCandidates);
     switch (KB) {
-    case 1: hipLaunchKernelGGL(mfmaCandidateKernel<1>, dim3(blocks), dim3(256), 0, stream(), ca); break;
-    case 2: hipLaunchKernelGGL(mfmaCandidateKernel<2>, dim3(blocks), dim3(256), 0, stream(), ca); break;
-    default: hipLaunchKernelGGL(mfmaCandidateKernel<4>, dim3(blocks), dim3(256), 0, stream(), ca); break;
+    case 1: hipLaunchKernelGGL(mfmaCandidateKernel<1>, dim3(blocks, splits), dim3(256), 0, stream(), ca); break;
+    case 2: hipLaunchKernelGGL(mfmaCandidateKernel<2>, dim3(blocks, splits), dim3(256), 0, stream(), ca); break;
+    default: hipLaunchKernelGGL(mfmaCandidateKernel<4>, dim3(blocks, splits), dim3(256), 0, stream(), ca); break;
     }
     s((int)hipGetLastError());
   }
