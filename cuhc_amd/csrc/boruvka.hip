@@ -25,6 +25,7 @@ int DevicePoints::upload(const float *hostPoints, int n, int d, bool spatial) {
   if (!spatial)
     return cuhc::upload(points, hostPoints, count * sizeof(float));
   order = spatialOrder(hostPoints, n, d);
+  host = hostPoints;
   DevArray<float> raw;
   s(raw.alloc(count));
   s(orig.alloc(n));
@@ -42,12 +43,13 @@ int DevicePoints::adopt(const float *devicePoints, int n, int d, bool spatial) {
     return 0;
   }
   const size_t count = (size_t)n * d;
-  std::vector<float> host(count);
+  hostCopy.resize(count);
   Status s;
-  s(download(host.data(), devicePoints, count * sizeof(float)));
+  s(download(hostCopy.data(), devicePoints, count * sizeof(float)));
   if (!s.ok())
     return s.code;
-  order = spatialOrder(host.data(), n, d);
+  host = hostCopy.data();
+  order = spatialOrder(host, n, d);
   s(points.alloc(count));
   s(orig.alloc(n));
   s(cuhc::upload(orig, order.data(), n * sizeof(int)));
@@ -63,7 +65,8 @@ int boruvkaMst(const float *X, int n, int d, MetricSpec m, WeightSpec w, const M
   s(ct.alloc(n));
   if (!s.ok() || n < 2)
     return s.code;
-  DevArray<int> dComp, dPosOf, dDone;
+  DevArray<int> dComp, dPosOf;
+  DevArray<unsigned int> dCompBest;
   DevArray<unsigned long long> dNearest;
   s(dComp.alloc(n));
   s(dNearest.alloc(n));
@@ -72,9 +75,9 @@ int boruvkaMst(const float *X, int n, int d, MetricSpec m, WeightSpec w, const M
     if (s.ok())
       s(inverseOrder(opt.devOrig, n, dPosOf));
   }
-  // Pruned passes reuse the previous round's rows that are still valid.
+  // Pruned passes bound each component's search by its best known edge.
   if (opt.prune)
-    s(dDone.alloc(n));
+    s(dCompBest.alloc(n));
   if (s.ok())
     s(resetComponents(dComp, n));
   MixedPass *mixed = nullptr;
@@ -91,17 +94,15 @@ int boruvkaMst(const float *X, int n, int d, MetricSpec m, WeightSpec w, const M
     if (mixed) {
       s(mixedPassRun(mixed, dComp, numComponents, m.id == kSqEuclidean, !haveResult, dNearest));
     } else {
-      const bool reuse = pruneThisPass && haveResult;
-      if (reuse)
-        s(markDoneRows(dNearest, dComp, posOf, n, dDone));
-      else
-        s(fill(dNearest, 0xff, n * sizeof(unsigned long long)));
+      if (pruneThisPass)
+        s(seedComponentBest(dNearest, dComp, posOf, n, numComponents, haveResult, dCompBest));
+      s(fill(dNearest, 0xff, n * sizeof(unsigned long long)));
       if (!s.ok())
         break;
       if (pruneThisPass) {
         double share = 1.0;
-        s(nearestOtherWalk(X, n, d, m, w, dComp, opt.devOrig, reuse ? dDone.get() : nullptr,
-                           dNearest, &share, opt.stats));
+        s(nearestOtherWalk(X, n, d, m, w, dComp, opt.devOrig, dCompBest, dNearest, &share,
+                           opt.stats));
         pruneThisPass = share <= kPruneGiveUpShare;
       } else {
         s(nearestOtherSymmetric(X, n, d, m, w, dComp, opt.devOrig, dNearest));
@@ -137,6 +138,41 @@ int singleLinkageMst(const DevicePoints &pts, int n, int d, MetricSpec m, bool p
   return status;
 }
 
+namespace {
+
+// Spatial order stratified by core distance: the points above the 90th
+// percentile of core distances (noise, in sparse regions) get their own k-d
+// order after the others'. A pruned pass skips a column tile when its bound
+// exceeds the largest current best of the row tile, and one sparse point's
+// large mutual-reachability weight would otherwise hold that bound up for the
+// dense points it shares a tile with. Returns the new order (original
+// indices) and new position -> old position, or empty vectors when one
+// class would be empty.
+void coreStratifiedOrder(const DevicePoints &pts, int n, int d, const std::vector<float> &core,
+                         std::vector<int> &order, std::vector<int> &newToOld) {
+  std::vector<float> sorted(core);
+  const size_t q = (size_t)(0.9 * n);
+  std::nth_element(sorted.begin(), sorted.begin() + q, sorted.end());
+  const float threshold = sorted[q];
+  std::vector<int> low, high; // original indices
+  for (int p = 0; p < n; ++p)
+    (core[p] > threshold ? high : low).push_back(pts.order[p]);
+  if (low.size() < 128 || high.size() < 128)
+    return;
+  spatialOrderIndices(pts.host, d, low.data(), (int)low.size());
+  spatialOrderIndices(pts.host, d, high.data(), (int)high.size());
+  order = low;
+  order.insert(order.end(), high.begin(), high.end());
+  std::vector<int> oldPos(n);
+  for (int p = 0; p < n; ++p)
+    oldPos[pts.order[p]] = p;
+  newToOld.resize(n);
+  for (int p = 0; p < n; ++p)
+    newToOld[p] = oldPos[order[p]];
+}
+
+} // namespace
+
 int hdbscanTree(const DevicePoints &pts, int n, int d, MetricSpec m, int minSamples, float alpha,
                 int coreMethod, bool prune, PassStats *stats, double *tree, float *hostCore) {
   DevArray<float> core;
@@ -145,22 +181,53 @@ int hdbscanTree(const DevicePoints &pts, int n, int d, MetricSpec m, int minSamp
   if (s.ok())
     s(coreDistances(pts.data(), n, d, m, minSamples, core, coreMethod,
                     prune && pts.ordered()));
+  // Pruned passes: reorder by core-distance class (the tree does not depend
+  // on the order).
+  const float *X = pts.data();
+  const int *devOrig = pts.ordered() ? pts.orig.get() : nullptr;
+  const std::vector<int> *order = &pts.order;
+  std::vector<int> stratified, newToOld;
+  DevArray<float> sPoints, sCore;
+  DevArray<int> sOrig, sMap;
+  if (s.ok() && prune && pts.ordered() && pts.host && n >= 1024) {
+    std::vector<float> hc(n);
+    s(download(hc.data(), core, n * sizeof(float)));
+    if (s.ok())
+      coreStratifiedOrder(pts, n, d, hc, stratified, newToOld);
+    if (s.ok() && !stratified.empty()) {
+      s(sPoints.alloc((size_t)n * d));
+      s(sCore.alloc(n));
+      s(sOrig.alloc(n));
+      s(sMap.alloc(n));
+      s(upload(sOrig, stratified.data(), n * sizeof(int)));
+      s(upload(sMap, newToOld.data(), n * sizeof(int)));
+      if (s.ok()) {
+        s(gatherRows(X, sMap, n, d, sPoints));
+        s(gatherRows(core, sMap, n, 1, sCore));
+      }
+      if (s.ok()) {
+        X = sPoints;
+        devOrig = sOrig;
+        order = &stratified;
+      }
+    }
+  }
   MstOptions opt;
   opt.prune = prune && pts.ordered();
-  opt.devOrig = pts.ordered() ? pts.orig.get() : nullptr;
+  opt.devOrig = devOrig;
   opt.stats = stats;
   WeightSpec w;
   w.kind = kWeightMReach;
-  w.core = core;
+  w.core = stratified.empty() ? core.get() : sCore.get();
   w.alpha = alpha;
   Contraction ct;
   if (s.ok())
-    s(boruvkaMst(pts.data(), n, d, m, w, opt, ct));
+    s(boruvkaMst(X, n, d, m, w, opt, ct));
   if (s.ok() && hostCore) {
     std::vector<float> c(n);
-    s(download(c.data(), core, n * sizeof(float)));
+    s(download(c.data(), w.core, n * sizeof(float)));
     for (int i = 0; i < n; ++i)
-      hostCore[pts.ordered() ? pts.order[i] : i] = c[i];
+      hostCore[pts.ordered() ? (*order)[i] : i] = c[i];
   }
   if (s.ok())
     s(treeFromDeviceEdges(ct.edges, n, tree));

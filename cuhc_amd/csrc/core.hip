@@ -57,16 +57,23 @@ DEV void insertSorted(float *list, int k, float v) {
   list[pos] = v;
 }
 
+// Rows are the points themselves (rowsX null: row tile blockIdx.x of X) or
+// listed points (rowsX: numRows gathered rows, rowPos their positions in X,
+// for the exact search of selected points).
 template <class T, class Metric>
 __global__ void __launch_bounds__(256)
     kthNearestFusedKernel(const float *__restrict__ X, int n, int d, Metric metric, int k,
-                          float *__restrict__ core, CoreBoxes boxes, int stepLimit) {
+                          float *__restrict__ core, CoreBoxes boxes, int stepLimit,
+                          const float *__restrict__ rowsX, int numRows,
+                          const int *__restrict__ rowPos) {
   constexpr int TR = T::TR;
   __shared__ __align__(16) TileSmem<T> sm;
   __shared__ int skip;
   extern __shared__ float lists[]; // [DIM][k], then the row tile's box (2d) if pruning
 
   const int row0 = blockIdx.x * T::DIM;
+  const float *A = rowsX ? rowsX : X;
+  const int nA = rowsX ? numRows : n;
   for (int i = threadIdx.x; i < T::DIM * k; i += T::THREADS)
     lists[i] = INFINITY;
   const bool prune = boxes.numTiles > 0;
@@ -81,10 +88,12 @@ __global__ void __launch_bounds__(256)
   const int numTiles = (n + T::DIM - 1) / T::DIM;
   const int lane = threadIdx.x & 63;
   const unsigned long long groupMask = 0xffffull << (lane & 48);
+  // Listed rows have no home tile: the walk starts anywhere and covers all.
+  const int homeTile = rowsX ? (int)(blockIdx.x % numTiles) : (int)blockIdx.x;
 
   const int steps = min(2 * numTiles, stepLimit);
   for (int step = 0; step < steps; ++step) {
-    const int colTile = walkTileCore(blockIdx.x, step, numTiles);
+    const int colTile = walkTileCore(homeTile, step, numTiles);
     if (colTile < 0)
       continue;
     if (prune) {
@@ -92,7 +101,7 @@ __global__ void __launch_bounds__(256)
       if (threadIdx.x < 64) {
         float largest = 0.0f;
         for (int i = threadIdx.x; i < T::DIM; i += 64)
-          if (row0 + i < n)
+          if (row0 + i < nA)
             largest = fmaxf(largest, lists[i * k + k - 1]);
 #pragma unroll
         for (int m = 32; m > 0; m >>= 1)
@@ -110,20 +119,21 @@ __global__ void __launch_bounds__(256)
     }
     const int col0 = colTile * T::DIM;
     float v[TR][TR];
-    computeTile<T>(sm, X, n, X, n, d, metric, row0, col0, v);
+    computeTile<T>(sm, A, nA, X, n, d, metric, row0, col0, v);
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
       const int rit = T::rowInTile(r);
       const int row = row0 + rit;
       float *list = lists + rit * k;
       unsigned int pending = 0;
-      if (row < n) {
+      if (row < nA) {
         const float kth = list[k - 1];
+        const int self = rowsX ? rowPos[row] : row;
 #pragma unroll
         for (int c = 0; c < TR; ++c) {
           const int col = col0 + T::colInTile(c);
           // The point itself is distance 0 (drops float noise such as cosine's 1e-7).
-          v[r][c] = col == row ? 0.0f : fmaxf(v[r][c], 0.0f);
+          v[r][c] = col == self ? 0.0f : fmaxf(v[r][c], 0.0f);
           if (col < n && v[r][c] < kth)
             pending |= 1u << c;
         }
@@ -153,8 +163,8 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   for (int i = threadIdx.x; i < T::DIM; i += T::THREADS)
-    if (row0 + i < n)
-      core[row0 + i] = lists[i * k + k - 1];
+    if (row0 + i < nA)
+      core[rowsX ? rowPos[row0 + i] : row0 + i] = lists[i * k + k - 1];
 }
 
 __global__ void coreBoxesKernel(const float *__restrict__ X, int n, int d, int tileDim,
@@ -256,7 +266,8 @@ int coreFused(const float *X, int n, int d, MetricSpec m, int k, float *core, bo
       }
       PhaseTimer timer(kPhaseCore);
       hipLaunchKernelGGL((kthNearestFusedKernel<T, Metric>), dim3(tiles), dim3(256), smem,
-                         stream(), X, n, d, metric, k, core, boxes, stepLimit);
+                         stream(), X, n, d, metric, k, core, boxes, stepLimit, nullptr, 0,
+                         nullptr);
       s((int)hipGetLastError());
       s(sync());
       return s.code;
@@ -295,10 +306,49 @@ int coreLocal(const float *X, int n, int d, MetricSpec m, int k, int stepLimit, 
   return coreFused(X, n, d, m, std::min(k, kMaxFusedK), out, false, stepLimit);
 }
 
+namespace {
+
+// Exact core distances of listed rows with the fused top-k lists (k <= 64):
+// the rows are gathered, every column tile is visited (no pruning: listed rows
+// are scattered).
+int coreForRowsFused(const float *X, int n, int d, MetricSpec m, int k, const int *rows,
+                     int numRows, float *core) {
+  DevArray<float> rowData;
+  Status s;
+  s(rowData.alloc((size_t)numRows * d));
+  if (s.ok())
+    s(gatherRows(X, rows, numRows, d, rowData));
+  if (!s.ok())
+    return s.code;
+  return withMetricCore(m, [&](auto metric) {
+    using Metric = decltype(metric);
+    auto run = [&](auto fill) {
+      using T = typename TileFor<Metric, decltype(fill)::value>::type;
+      const int rowTiles = (numRows + T::DIM - 1) / T::DIM;
+      const size_t smem = (size_t)T::DIM * k * sizeof(float);
+      PhaseTimer timer(kPhaseCore);
+      hipLaunchKernelGGL((kthNearestFusedKernel<T, Metric>), dim3(rowTiles), dim3(256), smem,
+                         stream(), X, n, d, metric, k, core, CoreBoxes{}, 0x7fffffff,
+                         rowData.get(), numRows, rows);
+      Status st;
+      st((int)hipGetLastError());
+      st(sync());
+      return st.code;
+    };
+    if (d <= 8)
+      return run(std::integral_constant<int, 8>{});
+    return run(std::integral_constant<int, 32>{});
+  });
+}
+
+} // namespace
+
 int coreForRows(const float *X, int n, int d, MetricSpec m, int k, const int *rows, int numRows,
                 float *core) {
   if (numRows <= 0)
     return 0;
+  if (k <= kMaxFusedK)
+    return coreForRowsFused(X, n, d, m, k, rows, numRows, core);
   const size_t maxBuffer = (size_t)512 << 20;
   const size_t rowBytes = (size_t)n * sizeof(float);
   int chunkRows = (int)std::max<size_t>(1, std::min<size_t>(maxBuffer / rowBytes, numRows));

@@ -10,6 +10,8 @@
 #include "tile.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <numeric>
 
 namespace cuhc {
@@ -277,16 +279,20 @@ DEV int walkTile(int rowTile, int step, int numTiles) {
 // One workgroup per row tile walks the column tiles outward from its own
 // (walkTile order). Lower bounds of the weight between the two boxes are
 // computed 64 steps at a time by wave 0, one step per lane, into LDS; a step
-// is skipped when its bound exceeds the largest current best of the tile's
-// open rows. Rows marked done (their nearest from the previous round is still
-// in another component, so still their nearest) are not open: they neither
-// need tiles nor hold the bound up, and a tile of done rows returns at once.
+// is skipped when its bound exceeds every row's need. A row needs candidates
+// below min(its current best, its component's best known edge): Boruvka only
+// uses each component's smallest outgoing edge, so once any point of the
+// component has found an edge of weight b, no row of that component needs a
+// candidate above b (compBest, shared by all workgroups through atomicMin on
+// the weight bits; null disables it). The row that holds the component's
+// smallest edge always finds it exactly; other rows may stop early, and their
+// results are then only upper bounds, which the contraction never prefers.
 template <class T, class Metric, class Weight>
 __global__ void __launch_bounds__(256)
     nearestOtherWalkKernel(const float *__restrict__ X, int n, int d, Metric metric,
                            Weight weight, const int *__restrict__ comp,
                            const int *__restrict__ orig, Boxes boxes,
-                           const int *__restrict__ done,
+                           unsigned int *__restrict__ compBest,
                            unsigned long long *__restrict__ nearest,
                            unsigned long long *tilesComputed, int rowTileStride) {
   constexpr int TR = T::TR;
@@ -295,7 +301,6 @@ __global__ void __launch_bounds__(256)
   __shared__ unsigned long long rowBest[T::DIM];
   __shared__ float lowerBound[kBatch];
   __shared__ float bound;
-  __shared__ int openRows;
   extern __shared__ float rowBox[]; // low[d], high[d]
 
   const int rowTile = blockIdx.x * rowTileStride, numTiles = boxes.numTiles;
@@ -306,18 +311,15 @@ __global__ void __launch_bounds__(256)
     rowBox[k] = boxes.low[(size_t)rowTile * d + k];
     rowBox[d + k] = boxes.high[(size_t)rowTile * d + k];
   }
-  // Wave 0 owns rows lane and lane + 64 for the bound reduction.
-  bool open0 = false, open1 = false;
-  if (threadIdx.x < 64) {
-    const int a = row0 + threadIdx.x, b = a + 64;
-    open0 = a < n && !(done && done[a]);
-    open1 = b < n && !(done && done[b]);
-    const unsigned long long any = __ballot(open0 || open1);
-    if (threadIdx.x == 0) {
-      bound = INFINITY;
-      openRows = any != 0;
-    }
-  }
+  // Wave 0 owns rows lane and lane + 64 for the bound: their components and
+  // the components' best known edges (refreshed once per batch).
+  const int ra = row0 + (int)(threadIdx.x & 63), rb = ra + 64;
+  const bool open0 = threadIdx.x < 64 && ra < n, open1 = threadIdx.x < 64 && rb < n;
+  const int comp0 = open0 ? comp[ra] : 0, comp1 = open1 ? comp[rb] : 0;
+  float cap0 = INFINITY, cap1 = INFINITY;
+  auto capOf = [&](unsigned int bits) { return bits == 0xffffffffu ? INFINITY : floatOf(bits); };
+  if (threadIdx.x == 0)
+    bound = INFINITY;
   const bool rowTilePure = boxes.minComp[rowTile] == boxes.maxComp[rowTile];
   const int rowCompTile = boxes.minComp[rowTile];
   const float rowMinValue = boxes.minValue[rowTile];
@@ -333,12 +335,48 @@ __global__ void __launch_bounds__(256)
   }
   unsigned long long computed = 0;
   __syncthreads();
-  if (!openRows)
-    return; // uniform
+
+  // The bound: the largest need over the open rows (inf while a row of a
+  // component without a known edge has no candidate yet). Wave 0 only.
+  auto newBound = [&]() {
+    float need = 0.0f;
+    if (open0) {
+      const unsigned long long k = rowBest[threadIdx.x];
+      const float w = k == kEmptyKey ? INFINITY : keyWeight(k);
+      if (compBest && w < cap0) {
+        atomicMin(&compBest[comp0], bitsOf(w));
+        cap0 = w;
+      }
+      need = fminf(w, cap0);
+    }
+    if (open1) {
+      const unsigned long long k = rowBest[threadIdx.x + 64];
+      const float w = k == kEmptyKey ? INFINITY : keyWeight(k);
+      if (compBest && w < cap1) {
+        atomicMin(&compBest[comp1], bitsOf(w));
+        cap1 = w;
+      }
+      need = fmaxf(need, fminf(w, cap1));
+    }
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1)
+      need = fmaxf(need, __shfl_xor(need, m, 64));
+    return need;
+  };
 
   const int steps = 2 * numTiles;
   for (int base = 0; base < steps; base += kBatch) {
     if (threadIdx.x < kBatch) {
+      if (compBest) {
+        // Other workgroups may have found better edges for these components.
+        if (open0)
+          cap0 = fminf(cap0, capOf(compBest[comp0]));
+        if (open1)
+          cap1 = fminf(cap1, capOf(compBest[comp1]));
+        const float b = newBound();
+        if (threadIdx.x == 0)
+          bound = b;
+      }
       // -1: never computed (outside the grid, or both tiles wholly one component).
       const int step = base + (int)threadIdx.x;
       const int colTile = step < steps ? walkTile(rowTile, step, numTiles) : -1;
@@ -390,21 +428,10 @@ __global__ void __launch_bounds__(256)
         }
       }
       __syncthreads();
-      // New bound: the largest current best over the open rows (inf while
-      // one has none yet).
       if (threadIdx.x < 64) {
-        float largest = 0.0f;
-        if (open0)
-          largest = rowBest[threadIdx.x] == kEmptyKey ? INFINITY : keyWeight(rowBest[threadIdx.x]);
-        if (open1)
-          largest = fmaxf(largest, rowBest[threadIdx.x + 64] == kEmptyKey
-                                       ? INFINITY
-                                       : keyWeight(rowBest[threadIdx.x + 64]));
-#pragma unroll
-        for (int m = 32; m > 0; m >>= 1)
-          largest = fmaxf(largest, __shfl_xor(largest, m, 64));
+        const float b = newBound();
         if (threadIdx.x == 0)
-          bound = largest;
+          bound = b;
       }
       __syncthreads();
     }
@@ -417,24 +444,43 @@ __global__ void __launch_bounds__(256)
     atomicAdd(tilesComputed, computed);
 }
 
-// Round r + 1 of Boruvka from round r's result: a point whose nearest (in
-// another component then) is still in another component keeps it (the other
-// points only became fewer); the rest start empty.
-__global__ void markDoneKernel(unsigned long long *__restrict__ nearest,
-                               const int *__restrict__ comp, const int *__restrict__ posOf, int n,
-                               int *__restrict__ done) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n)
-    return;
-  const unsigned long long key = nearest[p];
-  bool keep = false;
-  if (key != kEmptyKey) {
-    const int q = (int)keyIndex(key);
-    keep = comp[posOf ? posOf[q] : q] != comp[p];
+// Seeds each component's best known edge weight for the next pass from the
+// previous pass's results: a point whose previous candidate lies in another
+// component still has that edge. With few components the workgroup first
+// combines its points in LDS (one global atomic per component per workgroup).
+constexpr int kSeedLds = 4096;
+
+__global__ void __launch_bounds__(256)
+    seedComponentBestKernel(const unsigned long long *__restrict__ nearest,
+                            const int *__restrict__ comp, const int *__restrict__ posOf, int n,
+                            int numComp, unsigned int *compBest) {
+  __shared__ unsigned int local[kSeedLds];
+  const bool useLds = numComp <= kSeedLds;
+  if (useLds) {
+    for (int c = threadIdx.x; c < numComp; c += blockDim.x)
+      local[c] = 0xffffffffu;
+    __syncthreads();
   }
-  done[p] = keep;
-  if (!keep)
-    nearest[p] = kEmptyKey;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    const unsigned long long key = nearest[p];
+    if (key == kEmptyKey)
+      continue;
+    const int q = (int)keyIndex(key);
+    const int c = comp[p];
+    if (comp[posOf ? posOf[q] : q] == c)
+      continue; // the partner joined this component
+    const unsigned int bits = (unsigned int)(key >> 32);
+    if (useLds)
+      atomicMin(&local[c], bits);
+    else
+      atomicMin(&compBest[c], bits);
+  }
+  if (useLds) {
+    __syncthreads();
+    for (int c = threadIdx.x; c < numComp; c += blockDim.x)
+      if (local[c] != 0xffffffffu)
+        atomicMin(&compBest[c], local[c]);
+  }
 }
 
 __global__ void inverseOrderKernel(const int *__restrict__ orig, int n, int *__restrict__ posOf) {
@@ -548,44 +594,83 @@ bool shouldPrune(int mode, int metricId, int n, int d) {
 // is one tile (the behaviour of reference src/spatial.cuh:48-91). The order
 // inside the partitions of nth_element is implementation-defined; ties between
 // equal weights always break by original index, so results do not depend on it.
-std::vector<int> spatialOrder(const float *pts, int n, int d) {
+// The top levels run on one thread until there are enough ranges, then the
+// ranges are finished on up to 16 threads (disjoint ranges: the same order).
+void spatialOrderIndices(const float *pts, int d, int *idx, int count) {
   constexpr int kTile = 128, kSample = 64;
-  std::vector<int> order(n);
-  std::iota(order.begin(), order.end(), 0);
   struct Range {
     int begin, end;
   };
-  std::vector<Range> stack{{0, n}};
-  while (!stack.empty()) {
-    const Range range = stack.back();
-    stack.pop_back();
+  auto split = [&](Range range, std::vector<Range> &out) {
     const int length = range.end - range.begin;
     if (length <= kTile)
-      continue;
-    int split = 0;
+      return;
+    int feature = 0;
     float widest = -1.0f;
     const int step = std::max(1, length / kSample);
     for (int f = 0; f < d; ++f) {
       float lo = INFINITY, hi = -INFINITY;
       for (int i = range.begin; i < range.end; i += step) {
-        const float v = pts[(size_t)order[i] * d + f];
+        const float v = pts[(size_t)idx[i] * d + f];
         lo = std::min(lo, v);
         hi = std::max(hi, v);
       }
       if (hi - lo > widest) {
         widest = hi - lo;
-        split = f;
+        feature = f;
       }
     }
     const int tiles = (length + kTile - 1) / kTile;
     const int middle = range.begin + tiles / 2 * kTile;
-    std::nth_element(order.begin() + range.begin, order.begin() + middle,
-                     order.begin() + range.end, [&](int a, int b) {
-                       return pts[(size_t)a * d + split] < pts[(size_t)b * d + split];
-                     });
-    stack.push_back({range.begin, middle});
-    stack.push_back({middle, range.end});
+    std::nth_element(idx + range.begin, idx + middle, idx + range.end, [&](int x, int y) {
+      return pts[(size_t)x * d + feature] < pts[(size_t)y * d + feature];
+    });
+    out.push_back({range.begin, middle});
+    out.push_back({middle, range.end});
+  };
+  auto finish = [&](Range range) {
+    std::vector<Range> stack{range};
+    while (!stack.empty()) {
+      const Range r = stack.back();
+      stack.pop_back();
+      split(r, stack);
+    }
+  };
+  const unsigned threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  // Runs job(i) for i < count on the pool (disjoint ranges of idx).
+  auto parallel = [&](size_t jobs, auto &&job) {
+    std::atomic<size_t> cursor{0};
+    auto worker = [&]() {
+      for (size_t i = cursor++; i < jobs; i = cursor++)
+        job(i);
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < std::min<size_t>(threads, jobs); ++t)
+      pool.emplace_back(worker);
+    worker();
+    for (auto &t : pool)
+      t.join();
+  };
+  // Breadth first, one level at a time, each level's ranges in parallel,
+  // until there are enough ranges to finish them depth first in parallel.
+  std::vector<Range> level{{0, count}};
+  while (level.size() < 4 * threads) {
+    std::vector<std::vector<Range>> children(level.size());
+    parallel(level.size(), [&](size_t i) { split(level[i], children[i]); });
+    std::vector<Range> next;
+    for (const auto &c : children)
+      next.insert(next.end(), c.begin(), c.end());
+    if (next.empty())
+      return; // every range is one tile (ranges of one tile are final)
+    level.swap(next);
   }
+  parallel(level.size(), [&](size_t i) { finish(level[i]); });
+}
+
+std::vector<int> spatialOrder(const float *pts, int n, int d) {
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  spatialOrderIndices(pts, d, order.data(), n);
   return order;
 }
 
@@ -615,8 +700,8 @@ struct DeviceBoxes {
 
 template <class T, class Metric, class Weight>
 int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &weight,
-            const int *comp, const int *orig, const int *done, unsigned long long *nearest,
-            double *share, PassStats *stats, int stride) {
+            const int *comp, const int *orig, unsigned int *compBest,
+            unsigned long long *nearest, double *share, PassStats *stats, int stride) {
   const int tiles = (n + T::DIM - 1) / T::DIM;
   DeviceBoxes boxes;
   DevArray<unsigned long long> counter;
@@ -634,7 +719,7 @@ int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &we
     PhaseTimer timer(kPhaseDistanceTiles);
     hipLaunchKernelGGL((nearestOtherWalkKernel<T, Metric, Weight>), dim3(sampled), dim3(256),
                        2 * d * sizeof(float), stream(), X, n, d, metric, weight, comp, orig,
-                       boxes.view, done, nearest, counter.get(), stride);
+                       boxes.view, compBest, nearest, counter.get(), stride);
     s((int)hipGetLastError());
   }
   unsigned long long computed = 0;
@@ -652,7 +737,7 @@ int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &we
 } // namespace
 
 int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, const int *comp,
-                     const int *orig, const int *done, unsigned long long *nearest,
+                     const int *orig, unsigned int *compBest, unsigned long long *nearest,
                      double *share, PassStats *stats, int stride) {
   if (!metricHasBoxBound(m.id))
     return kErrInvalidMetric;
@@ -661,8 +746,8 @@ int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, c
       return kErrInvalidMetric;
     return withFill(d, [&](auto fill) {
       using T = Tile<8, decltype(fill)::value>;
-      return runWalk<T>(X, n, d, SqEuclidean{}, WardWeight{w.size}, comp, orig, done, nearest,
-                        share, stats, stride);
+      return runWalk<T>(X, n, d, SqEuclidean{}, WardWeight{w.size}, comp, orig, compBest,
+                        nearest, share, stats, stride);
     });
   }
   return withMetric(m, [&](auto metric) {
@@ -673,19 +758,25 @@ int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, c
       return withWeightFor(metric, w, [&](auto weight) {
         return withFill(d, [&](auto fill) {
           using T = Tile<8, decltype(fill)::value>;
-          return runWalk<T>(X, n, d, metric, weight, comp, orig, done, nearest, share, stats,
-                            stride);
+          return runWalk<T>(X, n, d, metric, weight, comp, orig, compBest, nearest, share,
+                            stats, stride);
         });
       });
     }
   });
 }
 
-int markDoneRows(unsigned long long *nearest, const int *comp, const int *posOf, int n,
-                 int *done) {
-  hipLaunchKernelGGL(markDoneKernel, dim3((n + 255) / 256), dim3(256), 0, stream(), nearest, comp,
-                     posOf, n, done);
-  return (int)hipGetLastError();
+int seedComponentBest(const unsigned long long *nearest, const int *comp, const int *posOf,
+                      int n, int numComp, bool fromNearest, unsigned int *compBest) {
+  Status st;
+  st(fill(compBest, 0xff, (size_t)numComp * sizeof(unsigned int)));
+  if (fromNearest) {
+    const int blocks = std::max(1, std::min(4 * numCUs(), (n + 255) / 256));
+    hipLaunchKernelGGL(seedComponentBestKernel, dim3(blocks), dim3(256), 0, stream(), nearest,
+                       comp, posOf, n, numComp, compBest);
+    st((int)hipGetLastError());
+  }
+  return st.code;
 }
 
 int inverseOrder(const int *orig, int n, int *posOf) {

@@ -21,6 +21,10 @@ struct DevicePoints {
   int adopt(const float *devicePoints, int n, int d, bool spatial);
   const float *data() const { return external ? external : points.get(); }
   const float *external = nullptr;
+  // Host coordinates in original order while ordered (the caller's array, or
+  // hostCopy for device-resident input).
+  const float *host = nullptr;
+  std::vector<float> hostCopy;
 };
 
 // Edges of a spanning forest in device memory (original indices, u < v).

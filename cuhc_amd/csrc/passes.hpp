@@ -55,6 +55,8 @@ bool shouldPrune(int mode, int metricId, int n, int d);
 
 // k-d order on the host: order[i] = original index of the point at position i.
 std::vector<int> spatialOrder(const float *hostPoints, int n, int d);
+// The same order of the listed points (indices into hostPoints), in place.
+void spatialOrderIndices(const float *hostPoints, int d, int *idx, int count);
 
 // dst[i] = src[order[i]] (rows of d floats).
 int gatherRows(const float *src, const int *order, int n, int d, float *dst);
@@ -65,20 +67,21 @@ struct PassStats {
 
 // Pruned pass for points in spatial order: same contract as
 // nearestOtherSymmetric, except that keys carry orig[j] (device array), so ties
-// break by original index exactly as in an unpruned run. done (nullable) marks
-// rows whose nearest[] already holds their answer (markDoneRows); they are not
-// searched. Returns the share of tile pairs computed in *share.
-// rowTileStride > 1 runs every stride-th row tile only (sampling).
+// break by original index exactly as in an unpruned run. compBest (nullable,
+// one u32 per component, seedComponentBest) holds the weight bits of the best
+// edge known for each component; rows stop searching above it, so a row's
+// result is exact only when it holds its component's smallest edge, which is
+// all the contraction uses. Returns the share of tile pairs computed in
+// *share. rowTileStride > 1 runs every stride-th row tile only (sampling).
 int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, const int *comp,
-                     const int *orig, const int *done, unsigned long long *nearest,
+                     const int *orig, unsigned int *compBest, unsigned long long *nearest,
                      double *share, PassStats *stats, int rowTileStride = 1);
 
-// Prepares nearest[] for the next Boruvka round from the last round's result
-// and the new components: rows whose nearest is still in another component
-// keep it and get done = 1; the others are reset to empty. posOf maps an
-// original index to its position (null: identity).
-int markDoneRows(unsigned long long *nearest, const int *comp, const int *posOf, int n,
-                 int *done);
+// compBest[c] for c < numComp: "none", then (fromNearest) the weight bits of
+// the previous pass's candidates whose partner is still in another component.
+// posOf maps an original index to its position (null: identity).
+int seedComponentBest(const unsigned long long *nearest, const int *comp, const int *posOf,
+                      int n, int numComp, bool fromNearest, unsigned int *compBest);
 
 // posOf[orig[p]] = p.
 int inverseOrder(const int *orig, int n, int *posOf);

@@ -110,3 +110,28 @@ def test_duplicates_and_single_point():
     np.testing.assert_array_equal(cuhc.HDBSCAN(min_cluster_size=5, min_samples=4)
                                   .fit(X).core_distances_, 0.0)
     assert np.all(cuhc.HDBSCAN(min_cluster_size=5, min_samples=5).fit(X).core_distances_ > 0)
+
+
+@pytest.mark.parametrize("metric", ["euclidean", "manhattan"])
+@pytest.mark.parametrize("k, alpha", [(1, 1.0), (10, 1.0), (25, 0.8)])
+def test_pruned_large_blobs_and_noise_identical(metric, k, alpha):
+    # 40,000 points: 12 blobs of unequal spread plus 15% uniform noise, so the
+    # pruned passes go through many rounds with a few large components, where
+    # rows stop at their component's best known edge (and the points are
+    # reordered by core-distance class); the unpruned symmetric pass is the
+    # reference.
+    rng = np.random.default_rng(40 + k)
+    centres = rng.uniform(-20, 20, (12, 5))
+    parts = [c + rng.normal(scale=s, size=(2_833, 5)) for c, s in
+             zip(centres, rng.uniform(0.3, 2.5, 12))]
+    X = np.concatenate(parts)
+    X = np.concatenate([X, rng.uniform(X.min(0), X.max(0), (6_000, 5))]).astype(np.float32)
+    out = {}
+    for mode in ["always", "never"]:
+        _lib.set_pruning(mode)
+        try:
+            out[mode] = hdbscan_tree(X, k, metric=metric, alpha=alpha)
+        finally:
+            _lib.set_pruning("auto")
+    np.testing.assert_array_equal(out["always"][1], out["never"][1])
+    np.testing.assert_array_equal(out["always"][0], out["never"][0])

@@ -204,3 +204,15 @@ def test_large_shuffled_line_with_tied_gaps(pruning):
     order = np.lexsort((hi, lo, gap))
     expected = kruskal_tree(n, [(gap[i], int(lo[i]), int(hi[i])) for i in order])
     np.testing.assert_array_equal(run(X, pruning=pruning), expected)
+
+
+@pytest.mark.parametrize("metric", ["euclidean", "chebyshev"])
+def test_pruned_large_clustered_identical(metric):
+    # 60,000 points in 4-d blobs of unequal spread: many pruned rounds with
+    # component bounds; the unpruned pass is the reference.
+    rng = np.random.default_rng(8)
+    centres = rng.uniform(-30, 30, (25, 4))
+    X = np.concatenate([c + rng.normal(scale=s, size=(2_400, 4))
+                        for c, s in zip(centres, rng.uniform(0.2, 3.0, 25))]).astype(np.float32)
+    np.testing.assert_array_equal(run(X, metric, pruning="always"),
+                                  run(X, metric, pruning="never"))
