@@ -11,21 +11,27 @@
 // (i*, fn(i*)) with i* the argmin over slots of (D(i, fn(i)), i). A merge
 // changes D(a, k) for every k and retires b, so only rows k < b can need a new
 // fn: rows with fn in {a, b} are rescanned unless the merged distance is still
-// the row's minimum; row a is always rescanned. Every rescan reads a
-// contiguous segment of row k; only the Lance-Williams update itself touches
-// the column a (strided).
+// the row's minimum; row a's new fn is the minimum of its new distances.
 //
-// The whole merge loop is one cooperative persistent kernel (one workgroup per
-// CU) with three grid barriers per merge:
-//   A  each workgroup takes the minimum over its slots
-//   B  all workgroups reduce those minima (redundantly) and know the pair;
-//   C  Lance-Williams update of row/column a over all k, fn maintenance, and
-//      rescan requests (row chunks) into a work list
-//   D  rescans: each chunk's minimum is folded into the row's scratch key
-// The tree row is written on the device; there is no host round trip.
-#include "linkage.hpp"
+// The merge loop is one cooperative persistent kernel (one workgroup per CU)
+// with two grid barriers per merge. Workgroup g owns a contiguous range of
+// slots (rows) and is the only writer of their fn, size, id and active flag,
+// and of D(k, a) / D(a, k) for its rows k. Merge s:
+//   update:  every workgroup reduces the per-workgroup selection records to
+//            the pair (a, b); applies the Lance-Williams update to D(a, k) for
+//            its own k, maintains its rows' fn, lists its rows that lost their
+//            nearest, and publishes its part of row a's new fn;
+//   barrier;
+//   rescan:  the owner of a reduces row a's parts; all workgroups share the
+//            listed rows' rescans in column chunks (atomicMin into fn), and
+//            each publishes the minimum (selection key, fn) over its own rows
+//            and the chunks it scanned;
+//   barrier.
+// Per-workgroup records replace same-address atomics, which serialize across
+// workgroups. The tree row is written on the device; there is no host round
+// trip between merges.
 
-#include <hip/hip_cooperative_groups.h>
+#include "linkage.hpp"
 
 #include <algorithm>
 
@@ -33,10 +39,16 @@ namespace cuhc {
 
 namespace {
 
-namespace cg = cooperative_groups;
-
-constexpr int kChunk = 8192;   // columns per rescan work item
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;
+constexpr int kChunk = 8192;                     // columns per initial-scan work item
+constexpr int kRescanChunk = 4096;               // columns per rescan work item
+constexpr int kRescanPer = kRescanChunk / kThreads;
+constexpr int kMaxOwned = 2048;                  // rows per workgroup (LDS rescan list)
+constexpr int kInlineRescans = 5;                // rescan rows carried in an update record
+constexpr int kRecordInts = 8;                   // update record: count, rows, padding
+constexpr int kGroup = 16;                       // workgroups per first-level barrier counter
+constexpr int kLineWords = 32;                   // one 128-byte line in 32-bit words
+constexpr unsigned int kSpinLimit = 1u << 22;    // barrier polls (seconds) before aborting
 
 // Order-preserving map of a float to 32 bits (all non-NaN values), so that
 // (distance, index) fits one u64 key without clamping negative distances.
@@ -70,6 +82,66 @@ DEV double lanceWilliams(int linkage, double dak, double dbk, double dab, double
   }
 }
 
+// Grid barrier for cooperatively launched (co-resident) workgroups: a
+// two-level counter tree, kGroup workgroups per first-level counter and one
+// root counter, each on its own line, and a generation word that the last
+// arrival advances. No counter sees more than max(kGroup, groups) contending
+// atomics. Counters only grow; the last arrival at a level is the one whose
+// increment completes a multiple of that level's membership.
+// Each wave waits for its own stores; one thread per workgroup then releases
+// at agent scope (L2 write-back), arrives, polls the generation with relaxed
+// loads (an acquire per poll would invalidate caches on every poll) and
+// acquires once. A wait longer than kSpinLimit polls, or an abort word set by
+// another workgroup, sets the abort word and returns false, so every
+// workgroup leaves the loop.
+struct Barrier {
+  unsigned int *words; // [kLineWords * (3 + groups)]: generation, abort, groups, root
+  int groups;
+};
+
+DEV unsigned int fetchAdd(unsigned int *p) {
+  return __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+DEV bool gridBarrier(const Barrier &bar, unsigned int &generation, int *ok) {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ++generation;
+    unsigned int *gen = bar.words;
+    unsigned int *abortWord = bar.words + kLineWords;
+    const int group = (int)blockIdx.x / kGroup;
+    const unsigned int members = (unsigned int)min(kGroup, (int)gridDim.x - group * kGroup);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned int old = fetchAdd(bar.words + kLineWords * (2 + group));
+    if ((old + 1) % members == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+      const unsigned int r = fetchAdd(bar.words + kLineWords * (2 + bar.groups));
+      if ((r + 1) % (unsigned int)bar.groups == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+        __hip_atomic_store(gen, generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    int result = 1;
+    for (unsigned int polls = 0;; ++polls) {
+      if (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == generation)
+        break;
+      if ((polls & 63) == 63 &&
+          (polls >= kSpinLimit ||
+           __hip_atomic_load(abortWord, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+        __hip_atomic_store(abortWord, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        result = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    *ok = result;
+  }
+  __syncthreads();
+  return *ok != 0;
+}
+
 struct DenseState {
   float *D;
   int n;
@@ -78,15 +150,43 @@ struct DenseState {
   int *size;
   int *id;
   unsigned char *active;
-  unsigned long long *fnKey;      // per slot: denseKey(D(i, fn(i)), fn(i)) or empty
-  unsigned long long *scratchKey; // per slot: rescan result being accumulated
-  unsigned long long *blockBest;  // per workgroup
-  int2 *items[2];                 // (row, first column) rescan work, by step parity
-  int *itemCount;                 // [2]
-  int itemCapacity;
-  double *tree; // (n - 1) x 4, device
-  unsigned long long *clocks; // null, or [3]: device-clock ticks in selection, update, rescan
+  unsigned long long *fnKey;       // per slot: denseKey(D(i, fn(i)), fn(i)) or empty
+  unsigned long long *selectSlots; // [2 G]: per workgroup (selection key, fn) of its best row
+  // Update records, one per workgroup: the minimum over its k > a of
+  // (D(a, k), k), and its rows to rescan (count, then up to kInlineRescans
+  // rows; all of them in rescanRows[g kMaxOwned ...] when there are more).
+  unsigned long long *rowASlots;   // [G]
+  int *records;                    // [G kRecordInts]
+  int *rescanRows;                 // [G kMaxOwned]
+  int2 *items;                     // initial fn: (row, first column) work items
+  double *tree;                    // (n - 1) x 4, device
+  unsigned long long *clocks;      // null, or [4]: device-clock ticks per part of a merge
+  Barrier bar;
 };
+
+// Exclusive prefix sum over the workgroup; total receives the sum.
+DEV int blockExclusiveScan(int v, int *waveSums, int &total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(x, d, 64);
+    if (lane >= d)
+      x += y;
+  }
+  __syncthreads();
+  if (lane == 63)
+    waveSums[wave] = x;
+  __syncthreads();
+  int before = 0, sum = 0;
+  for (int w = 0; w < kThreads / 64; ++w) {
+    const int sw = waveSums[w];
+    before += w < wave ? sw : 0;
+    sum += sw;
+  }
+  total = sum;
+  return before + x - v;
+}
 
 DEV unsigned long long blockMinKey(unsigned long long v, unsigned long long *red) {
 #pragma unroll
@@ -103,7 +203,35 @@ DEV unsigned long long blockMinKey(unsigned long long v, unsigned long long *red
   return r;
 }
 
-// Minimum over active j in [j0, j1) of denseKey(D(row, j), j), by the whole workgroup.
+// Lexicographic minimum of (key, payload) pairs.
+DEV void pairMin(unsigned long long &k, unsigned long long &p, unsigned long long k2,
+                 unsigned long long p2) {
+  if (k2 < k || (k2 == k && p2 < p)) {
+    k = k2;
+    p = p2;
+  }
+}
+
+DEV void blockMinPair(unsigned long long &k, unsigned long long &p, unsigned long long *redK,
+                      unsigned long long *redP) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1)
+    pairMin(k, p, shflXorKey(k, m), shflXorKey(p, m));
+  const int wave = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    redK[wave] = k;
+    redP[wave] = p;
+  }
+  __syncthreads();
+  k = redK[0];
+  p = redP[0];
+  for (int w = 1; w < kThreads / 64; ++w)
+    pairMin(k, p, redK[w], redP[w]);
+}
+
+// Minimum over active j in [j0, j1) of denseKey(D(row, j), j), by the whole
+// workgroup.
 DEV unsigned long long scanRowSegment(const DenseState &s, int row, int j0, int j1,
                                       unsigned long long *red) {
   unsigned long long best = kEmptyKey;
@@ -114,25 +242,32 @@ DEV unsigned long long scanRowSegment(const DenseState &s, int row, int j0, int 
   return blockMinKey(best, red);
 }
 
-// Enqueues the chunks of row r's forward segment into list p; resets its scratch key.
-DEV void enqueueRow(const DenseState &s, int p, int r) {
-  const int len = s.n - 1 - r;
-  if (len <= 0) {
-    s.scratchKey[r] = kEmptyKey;
-    return;
+// The same over one rescan chunk (at most kRescanChunk columns), with every
+// load issued before the first compare.
+DEV unsigned long long scanRescanChunk(const DenseState &s, int row, int j0, int j1,
+                                       unsigned long long *red) {
+  const size_t base = condensedIndex(row, row + 1, s.n) - (size_t)(row + 1);
+  float v[kRescanPer];
+  unsigned char act[kRescanPer];
+#pragma unroll
+  for (int u = 0; u < kRescanPer; ++u) {
+    const int j = j0 + (int)threadIdx.x + u * kThreads;
+    act[u] = j < j1 ? s.active[j] : 0;
+    v[u] = j < j1 ? s.D[base + j] : 0.0f;
   }
-  const int chunks = (len + kChunk - 1) / kChunk;
-  s.scratchKey[r] = kEmptyKey;
-  const int at = atomicAdd(&s.itemCount[p], chunks);
-  for (int c = 0; c < chunks && at + c < s.itemCapacity; ++c)
-    s.items[p][at + c] = make_int2(r, r + 1 + c * kChunk);
+  unsigned long long best = kEmptyKey;
+#pragma unroll
+  for (int u = 0; u < kRescanPer; ++u)
+    if (act[u])
+      best = minKey(best, denseKey(v[u], (unsigned)(j0 + (int)threadIdx.x + u * kThreads)));
+  return blockMinKey(best, red);
 }
 
 __global__ void __launch_bounds__(kThreads) denseInitKernel(DenseState s, int numItems) {
   __shared__ unsigned long long red[kThreads / 64];
   // Work item t: row r, chunk c, enumerated row by row.
   for (int t = blockIdx.x; t < numItems; t += gridDim.x) {
-    const int2 it = s.items[0][t];
+    const int2 it = s.items[t];
     const int j1 = min(s.n, it.y + kChunk);
     const unsigned long long k = scanRowSegment(s, it.x, it.y, j1, red);
     if (threadIdx.x == 0 && k != kEmptyKey)
@@ -140,144 +275,233 @@ __global__ void __launch_bounds__(kThreads) denseInitKernel(DenseState s, int nu
   }
 }
 
+// Selection key of a row: (distance of its forward nearest, row).
+DEV unsigned long long selectionKey(unsigned long long fn, int row) {
+  return fn == kEmptyKey ? kEmptyKey : (fn & 0xffffffff00000000ull) | (unsigned int)row;
+}
+
 __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
-  cg::grid_group grid = cg::this_grid();
-  __shared__ unsigned long long red[kThreads / 64];
-  __shared__ int shA, shB, shNa, shNb, shIdA, shIdB;
-  __shared__ float shDist;
+  __shared__ unsigned long long redK[kThreads / 64], redP[kThreads / 64];
+  __shared__ int ownRescan[kMaxOwned];
+  __shared__ int listStart[kThreads + 1];
+  __shared__ int listRows[kInlineRescans * kThreads];
+  __shared__ int waveSums[kThreads / 64];
+  __shared__ int ownRescanCount, barrierOk;
   const int n = s.n;
-  const int G = gridDim.x;
+  const int G = gridDim.x, g = blockIdx.x;
+  const int tid = threadIdx.x;
   const int per = (n + G - 1) / G;
-  const int mine0 = blockIdx.x * per, mine1 = min(n, mine0 + per);
-  const int gtid = blockIdx.x * kThreads + threadIdx.x, gthreads = G * kThreads;
-  const bool clocked = s.clocks && gtid == 0;
-  unsigned long long tSelect = 0, tUpdate = 0, tRescan = 0, t0 = 0, t1 = 0, t2 = 0;
+  const int mine0 = min(n, g * per), mine1 = min(n, mine0 + per);
+  const int maxChunks = (n - 1 + kRescanChunk - 1) / kRescanChunk;
+  unsigned int generation = 0;
+  const bool clocked = s.clocks && g == 0 && tid == 0;
+  unsigned long long ticks[4] = {0, 0, 0, 0}, t0 = 0;
+  auto lap = [&](int part) {
+    if (clocked) {
+      const unsigned long long t = wall_clock64();
+      ticks[part] += t - t0;
+      t0 = t;
+    }
+  };
+
+  // Selection records of merge 0 from the initial fn.
+  {
+    unsigned long long k = kEmptyKey, p = kEmptyKey;
+    for (int i = mine0 + tid; i < mine1; i += kThreads) {
+      const unsigned long long f = s.fnKey[i];
+      pairMin(k, p, selectionKey(f, i), f);
+    }
+    blockMinPair(k, p, redK, redP);
+    if (tid == 0) {
+      s.selectSlots[2 * g] = k;
+      s.selectSlots[2 * g + 1] = p;
+    }
+  }
+  if (!gridBarrier(s.bar, generation, &barrierOk))
+    return;
+  if (clocked)
+    t0 = wall_clock64();
 
   for (int step = 0; step < n - 1; ++step) {
-    const int p = step & 1, q = p ^ 1;
-    if (clocked)
-      t0 = wall_clock64();
-    // A: fold the previous step's rescans into fn, then this workgroup's minimum.
-    if (step > 0) {
-      const int cnt = min(s.itemCount[q], s.itemCapacity);
-      for (int t = threadIdx.x; t < cnt; t += kThreads) {
-        const int r = s.items[q][t].x;
-        if (r >= mine0 && r < mine1)
-          s.fnKey[r] = s.scratchKey[r];
-      }
-      __syncthreads();
+    // The pair: the lowest (selection key, fn) record.
+    if (tid == 0)
+      ownRescanCount = 0;
+    unsigned long long best = kEmptyKey, fa = kEmptyKey;
+    for (int t = tid; t < G; t += kThreads)
+      pairMin(best, fa, s.selectSlots[2 * t], s.selectSlots[2 * t + 1]);
+    blockMinPair(best, fa, redK, redP);
+    const int a = best == kEmptyKey ? -1 : keyIndex(best);
+    const int b = fa == kEmptyKey ? -1 : keyIndex(fa);
+    if (a < 0 || b <= a || b >= n) {
+      // No pair (non-finite distances): the remaining rows report -1. Every
+      // workgroup read the same records, so all of them stop here.
+      if (g == 0)
+        for (int r = step + tid; r < n - 1; r += kThreads)
+          for (int c = 0; c < 4; ++c)
+            s.tree[(size_t)r * 4 + c] = -1.0;
+      break;
     }
-    unsigned long long local = kEmptyKey;
-    for (int i = mine0 + (int)threadIdx.x; i < mine1; i += kThreads) {
-      const unsigned long long f = s.fnKey[i];
-      if (s.active[i] && f != kEmptyKey)
-        local = minKey(local, (f & 0xffffffff00000000ull) | (unsigned)i);
-    }
-    local = blockMinKey(local, red);
-    if (threadIdx.x == 0)
-      s.blockBest[blockIdx.x] = local;
-    grid.sync();
-
-    // B: the pair.
-    unsigned long long best = kEmptyKey;
-    for (int w = threadIdx.x; w < G; w += kThreads)
-      best = minKey(best, s.blockBest[w]);
-    best = blockMinKey(best, red);
-    if (threadIdx.x == 0) {
-      // With two or more active slots the lowest one always has a forward
-      // nearest; an empty key is reported (row of -1) and the step is skipped.
-      const int a = best == kEmptyKey ? -1 : keyIndex(best);
-      const unsigned long long fa = a >= 0 ? s.fnKey[a] : kEmptyKey;
-      const int b = fa == kEmptyKey ? -1 : keyIndex(fa);
-      shA = b >= 0 ? a : -1;
-      shB = b;
-      shDist = b >= 0 ? fromOrderedBits((unsigned)(fa >> 32)) : 0.0f;
-      shNa = b >= 0 ? s.size[a] : 0;
-      shNb = b >= 0 ? s.size[b] : 0;
-      shIdA = b >= 0 ? s.id[a] : -1;
-      shIdB = b >= 0 ? s.id[b] : -1;
-    }
-    __syncthreads();
-    if (clocked) {
-      t1 = wall_clock64();
-      tSelect += t1 - t0;
-    }
-    const int a = shA, b = shB, na = shNa, nb = shNb;
-    const float dab = shDist;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const float dab = fromOrderedBits((unsigned)(fa >> 32));
+    const int na = s.size[a], nb = s.size[b];
+    if (g == 0 && tid == 0) {
+      const int ia = s.id[a], ib = s.id[b];
       double *row = s.tree + (size_t)step * 4;
-      row[0] = (double)min(shIdA, shIdB);
-      row[1] = (double)max(shIdA, shIdB);
+      row[0] = (double)min(ia, ib);
+      row[1] = (double)max(ia, ib);
       row[2] = (double)dab;
       row[3] = (double)(na + nb);
-      s.itemCount[q] = 0; // list q was last read in phase A above
     }
-    if (a < 0) { // uniform: every workgroup reduced the same minima
-      grid.sync();
-      grid.sync();
-      continue;
-    }
+    lap(0);
 
-    // C: Lance-Williams update of D(a, k) for all active k, fn maintenance.
-    for (int k = gtid; k < n; k += gthreads) {
-      if (k == a || k == b || !s.active[k])
+    // Lance-Williams update of D(a, k) for this workgroup's k, its rows' fn,
+    // and its part of row a's new fn.
+    unsigned long long rowAMin = kEmptyKey, selK = kEmptyKey, selP = kEmptyKey;
+    for (int k = mine0 + tid; k < mine1; k += kThreads) {
+      if (k == a || k == b)
         continue;
+      // Every load is issued before the activity test: one round trip.
       const size_t iak = k < a ? condensedIndex(k, a, n) : condensedIndex(a, k, n);
       const size_t ibk = k < b ? condensedIndex(k, b, n) : condensedIndex(b, k, n);
+      const unsigned char act = s.active[k];
       const float dak = s.D[iak], dbk = s.D[ibk];
-      const float nd = (float)lanceWilliams(s.linkage, dak, dbk, dab, na, nb, s.size[k]);
+      const unsigned long long f = s.fnKey[k];
+      const int nk = s.size[k];
+      if (!act)
+        continue;
+      const float nd = (float)lanceWilliams(s.linkage, dak, dbk, dab, na, nb, nk);
       s.D[iak] = nd;
+      unsigned long long nf = f;
+      bool rescan = false;
       if (k < a) {
-        const unsigned long long f = s.fnKey[k];
-        const int fk = keyIndex(f);
+        // Every entry of row k but D(k, a) and D(k, b) is unchanged.
+        const int fk = f == kEmptyKey ? -1 : keyIndex(f);
         const unsigned long long cand = denseKey(nd, (unsigned)a);
-        if (f != kEmptyKey && (fk == a || fk == b)) {
-          // The merged cluster replaces the row's nearest if it is no farther;
-          // every other entry of the row is unchanged and was >= the old minimum.
+        if (fk == a || fk == b) {
+          // The other entries were >= the old minimum, and any equal one has
+          // a higher column than b: (nd, a) is the minimum if nd is no larger.
           if (orderedBits(nd) <= (unsigned)(f >> 32))
-            s.fnKey[k] = cand;
+            nf = cand;
           else
-            enqueueRow(s, p, k);
+            rescan = true;
         } else if (cand < f) {
-          s.fnKey[k] = cand;
+          nf = cand;
         }
-      } else if (k < b) {
-        if (keyIndex(s.fnKey[k]) == b && s.fnKey[k] != kEmptyKey)
-          enqueueRow(s, p, k); // its nearest retired
+      } else {
+        rowAMin = minKey(rowAMin, denseKey(nd, (unsigned)k));
+        if (k < b && f != kEmptyKey && keyIndex(f) == b)
+          rescan = true; // its nearest retired
       }
+      if (rescan) {
+        nf = kEmptyKey; // rebuilt by atomicMin over the chunks
+        ownRescan[atomicAdd(&ownRescanCount, 1)] = k;
+      } else {
+        pairMin(selK, selP, selectionKey(nf, k), nf);
+      }
+      if (nf != f)
+        s.fnKey[k] = nf;
     }
-    if (gtid == 0) {
+    rowAMin = blockMinKey(rowAMin, redK); // also orders the ownRescan appends
+    const int own = ownRescanCount;
+    int *record = s.records + (size_t)g * kRecordInts;
+    if (tid == 0) {
+      s.rowASlots[g] = rowAMin;
+      record[0] = own;
+    }
+    if (tid < min(own, kInlineRescans))
+      record[1 + tid] = ownRescan[tid];
+    if (own > kInlineRescans)
+      for (int i = tid; i < own; i += kThreads)
+        s.rescanRows[(size_t)g * kMaxOwned + i] = ownRescan[i];
+    if (tid == 0 && b >= mine0 && b < mine1) {
       s.active[b] = 0;
       s.fnKey[b] = kEmptyKey;
-      enqueueRow(s, p, a);
     }
-    grid.sync();
-    if (clocked) {
-      t2 = wall_clock64();
-      tUpdate += t2 - t1;
-    }
+    lap(1);
+    if (!gridBarrier(s.bar, generation, &barrierOk))
+      return;
+    lap(2);
 
-    // D: rescans; bookkeeping of the merged slot.
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      s.size[a] = na + nb;
-      s.id[a] = s.firstNewId + step;
+    // Every workgroup's update record (G <= kThreads: one per thread).
+    unsigned long long rowAPart = kEmptyKey;
+    int cnt = 0;
+    int inlineRows[kInlineRescans];
+    if (tid < G) {
+      rowAPart = s.rowASlots[tid];
+      const int4 r0 = *reinterpret_cast<const int4 *>(s.records + (size_t)tid * kRecordInts);
+      const int4 r1 = *reinterpret_cast<const int4 *>(s.records + (size_t)tid * kRecordInts + 4);
+      cnt = r0.x;
+      inlineRows[0] = r0.y;
+      inlineRows[1] = r0.z;
+      inlineRows[2] = r0.w;
+      inlineRows[3] = r1.x;
+      inlineRows[4] = r1.y;
     }
-    const int cnt = min(s.itemCount[p], s.itemCapacity);
-    for (int t = blockIdx.x; t < cnt; t += G) {
-      const int2 it = s.items[p][t];
-      const unsigned long long k = scanRowSegment(s, it.x, it.y, min(n, it.y + kChunk), red);
-      if (threadIdx.x == 0 && k != kEmptyKey)
-        atomicMin(&s.scratchKey[it.x], k);
+    // Row a's new fn, from every workgroup's part.
+    if (a >= mine0 && a < mine1) {
+      const unsigned long long m = blockMinKey(rowAPart, redK);
+      if (tid == 0) {
+        s.fnKey[a] = m;
+        s.size[a] = na + nb;
+        s.id[a] = s.firstNewId + step;
+        pairMin(selK, selP, selectionKey(m, a), m);
+      }
     }
-    grid.sync();
-    if (clocked)
-      tRescan += wall_clock64() - t2;
+    // The rescan list: workgroup w's rows at [listStart[w], listStart[w + 1]).
+    int count = 0;
+    const int start = blockExclusiveScan(cnt, waveSums, count);
+    const bool spilled = __syncthreads_or(cnt > kInlineRescans);
+    if (tid < G) {
+      listStart[tid] = start;
+      if (!spilled)
+        for (int i = 0; i < cnt; ++i)
+          listRows[start + i] = inlineRows[i];
+    }
+    if (tid == 0)
+      listStart[G] = count;
+    __syncthreads();
+    // Rescans: item t is chunk t % maxChunks of list row t / maxChunks.
+    const long long numItems = (long long)count * maxChunks;
+    for (long long t = g; t < numItems; t += G) {
+      const int q = (int)(t / maxChunks);
+      int r;
+      if (!spilled) {
+        r = listRows[q];
+      } else {
+        int lo = 0, hi = G; // the w with listStart[w] <= q < listStart[w + 1]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (listStart[mid] <= q)
+            lo = mid;
+          else
+            hi = mid;
+        }
+        const int i = q - listStart[lo];
+        r = listStart[lo + 1] - listStart[lo] <= kInlineRescans
+                ? s.records[(size_t)lo * kRecordInts + 1 + i]
+                : s.rescanRows[(size_t)lo * kMaxOwned + i];
+      }
+      const int j0 = r + 1 + (int)(t % maxChunks) * kRescanChunk;
+      if (j0 >= n)
+        continue;
+      const unsigned long long m = scanRescanChunk(s, r, j0, min(n, j0 + kRescanChunk), redK);
+      if (tid == 0 && m != kEmptyKey) {
+        atomicMin(&s.fnKey[r], m);
+        pairMin(selK, selP, selectionKey(m, r), m);
+      }
+    }
+    blockMinPair(selK, selP, redK, redP);
+    if (tid == 0) {
+      s.selectSlots[2 * g] = selK;
+      s.selectSlots[2 * g + 1] = selP;
+    }
+    lap(3);
+    if (!gridBarrier(s.bar, generation, &barrierOk))
+      return;
+    lap(2);
   }
-  if (clocked) {
-    s.clocks[0] = tSelect;
-    s.clocks[1] = tUpdate;
-    s.clocks[2] = tRescan;
-  }
+  if (clocked)
+    for (int i = 0; i < 4; ++i)
+      s.clocks[i] = ticks[i];
 }
 
 __global__ void denseSetupKernel(DenseState s, const int *initialSize, const int *initialId) {
@@ -286,7 +510,6 @@ __global__ void denseSetupKernel(DenseState s, const int *initialSize, const int
     s.id[i] = initialId ? initialId[i] : i;
     s.active[i] = 1;
     s.fnKey[i] = kEmptyKey;
-    s.scratchKey[i] = kEmptyKey;
   }
 }
 
@@ -312,30 +535,33 @@ int denseLinkageOnCondensed(float *D, int n, int linkage, const int *initialSize
                             const int *initialId, int firstNewId, double *hostTree) {
   if (n < 2)
     return 0;
+  const int G = numCUs();
+  if ((n + G - 1) / G > kMaxOwned || G > kThreads)
+    return kErrInvalidShape; // beyond any condensed matrix that fits in HBM
   DenseState st{};
-  DevArray<int> size, id, itemCount;
+  DevArray<int> size, id, records, rescanRows;
   DevArray<unsigned char> active;
-  DevArray<unsigned long long> fnKey, scratchKey, blockBest;
-  DevArray<int2> items0, items1, initItems;
+  DevArray<unsigned long long> fnKey, selectSlots, rowASlots;
+  DevArray<int2> initItems;
   DevArray<double> tree;
   DevArray<unsigned long long> clocks;
+  DevArray<unsigned int> barrierWords;
   Status s;
   const int maxChunks = (n - 1 + kChunk - 1) / kChunk;
   const long long initCount = (long long)(n - 1) * maxChunks;
-  // Per step at most one enqueue per row; a row needs at most maxChunks items.
-  const long long cap = std::min<long long>((long long)n * maxChunks, 1LL << 30);
-  const int G = numCUs();
+  const int groups = (G + kGroup - 1) / kGroup;
+  const size_t numBarrierWords = (size_t)kLineWords * (3 + groups);
   s(size.alloc(n));
   s(id.alloc(n));
-  s(itemCount.alloc(2));
   s(active.alloc(n));
   s(fnKey.alloc(n));
-  s(scratchKey.alloc(n));
-  s(blockBest.alloc(G));
-  s(items0.alloc(cap));
-  s(items1.alloc(cap));
+  s(selectSlots.alloc(2 * (size_t)G));
+  s(rowASlots.alloc(G));
+  s(records.alloc((size_t)G * kRecordInts));
+  s(rescanRows.alloc((size_t)G * kMaxOwned));
   s(initItems.alloc(initCount));
   s(tree.alloc((size_t)(n - 1) * 4));
+  s(barrierWords.alloc(numBarrierWords));
   if (!s.ok())
     return s.code;
   st.D = D;
@@ -346,29 +572,29 @@ int denseLinkageOnCondensed(float *D, int n, int linkage, const int *initialSize
   st.id = id;
   st.active = active;
   st.fnKey = fnKey;
-  st.scratchKey = scratchKey;
-  st.blockBest = blockBest;
-  st.items[0] = items0;
-  st.items[1] = items1;
-  st.itemCount = itemCount;
-  st.itemCapacity = (int)cap;
+  st.selectSlots = selectSlots;
+  st.rowASlots = rowASlots;
+  st.records = records;
+  st.rescanRows = rescanRows;
+  st.items = initItems;
   st.tree = tree;
   st.clocks = nullptr;
+  st.bar.words = barrierWords;
+  st.bar.groups = groups;
   if (timingEnabled()) {
-    s(clocks.alloc(3));
-    s(fill(clocks, 0, 3 * sizeof(unsigned long long)));
+    s(clocks.alloc(4));
+    s(fill(clocks, 0, 4 * sizeof(unsigned long long)));
     st.clocks = clocks;
   }
-  s(fill(itemCount, 0, 2 * sizeof(int)));
+  s(fill(barrierWords, 0, numBarrierWords * sizeof(unsigned int)));
   PhaseTimer timer(kPhaseDense);
   hipLaunchKernelGGL(denseSetupKernel, dim3(std::min(1024, (n + 255) / 256)), dim3(256), 0,
                      stream(), st, initialSize, initialId);
   hipLaunchKernelGGL(denseInitItemsKernel, dim3((n + 255) / 256), dim3(256), 0, stream(),
                      initItems.get(), n);
-  DenseState initState = st;
-  initState.items[0] = initItems;
-  hipLaunchKernelGGL(denseInitKernel, dim3((unsigned)std::max<long long>(1, std::min<long long>(initCount, 8LL * G))),
-                     dim3(kThreads), 0, stream(), initState, (int)initCount);
+  hipLaunchKernelGGL(denseInitKernel,
+                     dim3((unsigned)std::max<long long>(1, std::min<long long>(initCount, 8LL * G))),
+                     dim3(kThreads), 0, stream(), st, (int)initCount);
   s((int)hipGetLastError());
   if (s.ok()) {
     void *args[] = {&st};
@@ -376,12 +602,19 @@ int denseLinkageOnCondensed(float *D, int n, int linkage, const int *initialSize
                                       args, 0, stream()));
   }
   s(sync());
+  unsigned int aborted = 0;
+  if (s.ok())
+    s(download(&aborted, barrierWords.get() + kLineWords, sizeof(aborted)));
+  if (s.ok() && aborted)
+    return (int)hipErrorLaunchTimeOut; // a barrier wait exceeded its limit
   if (s.ok() && st.clocks) {
-    unsigned long long ticks[3];
+    unsigned long long ticks[4];
     s(download(ticks, clocks, sizeof(ticks)));
     const double khz = wallClockKHz();
-    for (int i = 0; i < 3; ++i)
-      addMs(kPhaseDenseSelect + i, (double)ticks[i] / khz, 1);
+    addMs(kPhaseDenseSelect, (double)ticks[0] / khz, 1);
+    addMs(kPhaseDenseUpdate, (double)ticks[1] / khz, 1);
+    addMs(kPhaseDenseBarrier, (double)ticks[2] / khz, 1);
+    addMs(kPhaseDenseRescan, (double)ticks[3] / khz, 1);
   }
   if (s.ok())
     s(download(hostTree, tree, (size_t)(n - 1) * 4 * sizeof(double)));

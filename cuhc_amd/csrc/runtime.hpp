@@ -70,10 +70,11 @@ enum Phase : int {
   kPhaseDense = 5,          // dense Lance-Williams loop
   kPhaseRnn = 6,            // RNN merge/compaction kernels
   kPhaseMlp = 7,            // learned-radius network
-  kPhaseDenseSelect = 8,    // dense loop, device clock: pair selection (phases A, B)
-  kPhaseDenseUpdate = 9,    // dense loop, device clock: Lance-Williams update (C)
-  kPhaseDenseRescan = 10,   // dense loop, device clock: row rescans (D)
-  kNumPhases = 11,
+  kPhaseDenseSelect = 8,    // dense loop, device clock: pair selection
+  kPhaseDenseUpdate = 9,    // dense loop, device clock: Lance-Williams update
+  kPhaseDenseRescan = 10,   // dense loop, device clock: row rescans and selection records
+  kPhaseDenseBarrier = 11,  // dense loop, device clock: grid barriers
+  kNumPhases = 12,
 };
 
 // Times the enclosed launches of one phase; a timer nested inside another of

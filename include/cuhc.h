@@ -92,8 +92,8 @@ int cuhc_copy_to_host(void *dst, const void *src, unsigned long long bytes);
 /* Per-phase GPU timing with HIP events on the library stream (0 = off).
  * Phases: 0 FP32 distance tiles, 1 FP16 MFMA candidates, 2 FP32 recheck,
  * 3 core distances, 4 component bookkeeping, 5 dense loop, 6 RNN kernels,
- * 7 learned-radius network; 8-10 split the dense loop by the device clock
- * into pair selection, Lance-Williams update and row rescans. */
+ * 7 learned-radius network; 8-11 split the dense loop by the device clock
+ * into pair selection, Lance-Williams update, row rescans and grid barriers. */
 int cuhc_set_timing(int on);
 /* Fills up to count phases: total ms, launches, algorithmic units (pairs). */
 void cuhc_read_timings(double *ms, long long *launches, long long *units, int count);

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 from scipy.cluster.hierarchy import fcluster as scipy_fcluster
 from scipy.cluster.hierarchy import linkage as scipy_linkage
+from scipy.spatial.distance import pdist
 
 import cuhc_amd as cuhc
 import oracle
@@ -55,6 +56,23 @@ def test_dense_larger_and_duplicates():
         assert_valid_tree(Z, 200)
         np.testing.assert_allclose(Z[:160, 2], 0.0, atol=1e-5)
         assert Z[160, 2] > 1e-5
+
+
+@pytest.mark.parametrize("method", ["single", "complete", "average"])
+def test_dense_star_many_rescans(method):
+    # Random unit vectors and the origin last: every row's nearest is the
+    # origin (distance about 1, all other pairs farther), so the first merge
+    # leaves every row without its nearest: every slot range lists more rows
+    # to rescan than an update record carries inline (1401 rows over at most
+    # 256 workgroups is 6 per workgroup).
+    m = 1400
+    rng = np.random.default_rng(5)
+    V = rng.normal(size=(m, 128))
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    X = np.vstack([V, np.zeros((1, 128))]).astype(np.float32)
+    assert pdist(X[:m]).min() > np.linalg.norm(X[:m], axis=1).max()
+    Z = cuhc.linkage(X, method=method, algorithm="dense")
+    np.testing.assert_array_equal(Z, oracle.dense_linkage(X, method))
 
 
 @pytest.mark.parametrize("n", [2, 3, 127, 128, 129, 600])
