@@ -374,7 +374,10 @@ DEV void ldsDirectLoad4(const void *g, const void *lds) {
                : "memory");
 }
 
-template <int KB>
+// SPLIT: several workgroups (gridDim.y column splits) share each row block.
+// A separate instantiation keeps the atomic list position, whose return the
+// wave must wait for, out of the unsplit kernel.
+template <int KB, bool SPLIT>
 __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
   constexpr int DP = 16 * KB;
   constexpr int TILE_BYTES = kTileCols * DP * 2;
@@ -479,7 +482,7 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
   // Column split blockIdx.y of gridDim.y takes an equal share of the tiles
   // (several workgroups per row block when rows are few).
   const int splits = gridDim.y;
-  const bool shared = splits > 1;
+  constexpr bool shared = SPLIT;
   const int it0 = (int)((long long)allIters * blockIdx.y / splits);
   const int iters = (int)((long long)allIters * (blockIdx.y + 1) / splits) - it0;
   auto tileOf = [&](int i) {
@@ -1050,12 +1053,18 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
     s(fill(mp->candCount, 0, (size_t)numRows * 2 * sizeof(int)));
     s(fill(mp->rowMin, 0xff, (size_t)numRows * sizeof(unsigned int)));
   }
+  auto launch = [&](auto unsplit, auto split) {
+    if (splits > 1)
+      hipLaunchKernelGGL(split, dim3(blocks, splits), dim3(256), 0, stream(), ca);
+    else
+      hipLaunchKernelGGL(unsplit, dim3(blocks), dim3(256), 0, stream(), ca);
+  };
   {
     PhaseTimer timer(kPhaseMfmaCandidates);
     switch (KB) {
-    case 1: hipLaunchKernelGGL(mfmaCandidateKernel<1>, dim3(blocks, splits), dim3(256), 0, stream(), ca); break;
-    case 2: hipLaunchKernelGGL(mfmaCandidateKernel<2>, dim3(blocks, splits), dim3(256), 0, stream(), ca); break;
-    default: hipLaunchKernelGGL(mfmaCandidateKernel<4>, dim3(blocks, splits), dim3(256), 0, stream(), ca); break;
+    case 1: launch(mfmaCandidateKernel<1, false>, mfmaCandidateKernel<1, true>); break;
+    case 2: launch(mfmaCandidateKernel<2, false>, mfmaCandidateKernel<2, true>); break;
+    default: launch(mfmaCandidateKernel<4, false>, mfmaCandidateKernel<4, true>); break;
     }
     s((int)hipGetLastError());
   }
