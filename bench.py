@@ -43,6 +43,39 @@ def tree_on_device(lib, ptr, n, d, tree):
     _lib.check(lib.cuhc_linkage_device(ptr, n, d, 0, 2.0, 0, 1, _lib.double_ptr(tree)))
 
 
+PROFILE_C4 = ROOT / "profiles" / "c4_single_linkage_1M_d32_mixed.md"
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the candidate kernel, from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (its 'traffic_bytes_per_launch' line), or None."""
+    try:
+        for line in PROFILE_C4.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def calibrate_copy(mib):
+    """One device-to-device copy of mib MiB (a plain copy kernel of known size,
+    run under rocprofv3 --pmc to scale FETCH_SIZE / WRITE_SIZE)."""
+    import ctypes
+    from cuhc_amd import _lib
+    nbytes = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.check(_lib.lib.cuhc_device_alloc(nbytes, ctypes.byref(src)))
+    _lib.check(_lib.lib.cuhc_device_alloc(nbytes, ctypes.byref(dst)))
+    try:
+        for _ in range(2):
+            _lib.check(_lib.lib.cuhc_copy_device(dst, src, nbytes))
+    finally:
+        _lib.lib.cuhc_device_free(src)
+        _lib.lib.cuhc_device_free(dst)
+    print(json.dumps({"copy_kib_per_launch": nbytes >> 10, "launches": 2}), flush=True)
+
+
 def timed_steps(fn, steps):
     # Each call is blocking: it returns after the tree is in host memory and
     # the library's stream is idle, so wall time brackets exactly the steps.
@@ -156,6 +189,8 @@ def main():
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--core-methods", default="auto,learned",
                         help="c3: comma-separated core-distance methods to time")
+    parser.add_argument("--calibrate-copy-mib", type=int, default=0,
+                        help="only run a device-to-device copy of this size (counter scaling)")
     args = parser.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); "
@@ -165,6 +200,9 @@ def main():
     import cuhc_amd  # noqa: F401
     from cuhc_amd import _lib
 
+    if args.calibrate_copy_mib > 0:
+        calibrate_copy(args.calibrate_copy_mib)
+        return
     if args.config != "c4":
         other_config(args.config, args)
         return
@@ -208,7 +246,7 @@ def main():
         "roofline": {"bound": "mfma", "kernel": "mfmaCandidateKernel<2>",
                      "achieved": round(achieved, 2), "peak": FP16_MFMA_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP16_MFMA_PEAK_TFLOPS, 4),
-                     "traffic": None, "avg_kernel_ms": round(kernel_ms, 3),
+                     "traffic": profiled_traffic(), "avg_kernel_ms": round(kernel_ms, 3),
                      "launches_per_tree": launches / max(args.steps, 1),
                      "algorithmic_flops_per_launch": flops_per_launch},
         "phase_ms_per_tree": {k: round(v[0] / max(args.steps, 1), 3) for k, v in phases.items()

@@ -52,6 +52,7 @@ _declare("cuhc_hdbscan_tree_device", _int, ctypes.c_void_p, _int, _int, _int, _f
 _declare("cuhc_device_alloc", _int, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_void_p))
 _declare("cuhc_device_free", _int, ctypes.c_void_p)
 _declare("cuhc_copy_to_device", _int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong)
+_declare("cuhc_copy_device", _int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong)
 _declare("cuhc_copy_to_host", _int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong)
 _declare("cuhc_set_timing", _int, _int)
 _declare("cuhc_read_timings", None, _f64p, _i64p, _i64p, _int)

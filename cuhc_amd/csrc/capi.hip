@@ -253,6 +253,11 @@ int cuhc_copy_to_host(void *dst, const void *src, unsigned long long bytes) {
   return download(dst, src, bytes);
 }
 
+int cuhc_copy_device(void *dst, const void *src, unsigned long long bytes) {
+  int s = (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream());
+  return s ? s : sync();
+}
+
 int cuhc_set_timing(int on) {
   setTiming(on != 0);
   return 0;

@@ -53,6 +53,31 @@ __global__ void componentMinKernel(const unsigned long long *__restrict__ neares
   atomicMin(&best1[comp[p]], lowKey(key, orig ? orig[p] : p));
 }
 
+// The same with few components: same-address atomics from many waves
+// serialise (one component holding every point would take one atomic per
+// point), so each workgroup first reduces into LDS and then flushes one atomic
+// per component it saw.
+constexpr int kLdsComponents = 4096;
+
+__global__ void __launch_bounds__(256)
+    componentMinLdsKernel(const unsigned long long *__restrict__ nearest,
+                          const int *__restrict__ comp, const int *__restrict__ orig, int n,
+                          int numComp, unsigned long long *best1) {
+  __shared__ unsigned long long local[kLdsComponents];
+  for (int c = threadIdx.x; c < numComp; c += blockDim.x)
+    local[c] = kEmptyKey;
+  __syncthreads();
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    const unsigned long long key = nearest[p];
+    if (key != kEmptyKey)
+      atomicMin(&local[comp[p]], lowKey(key, orig ? orig[p] : p));
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < numComp; c += blockDim.x)
+    if (local[c] != kEmptyKey)
+      atomicMin(&best1[c], local[c]);
+}
+
 __global__ void componentMin2Kernel(const unsigned long long *__restrict__ nearest,
                                     const int *__restrict__ comp, const int *__restrict__ orig,
                                     int n, const unsigned long long *__restrict__ best1,
@@ -183,8 +208,12 @@ int contractRound(Contraction &ct, const unsigned long long *nearest, int *comp,
   Status s;
   hipLaunchKernelGGL(resetComponentsKernel, gridFor(numComp), dim3(256), 0, stream(),
                      ct.best1.get(), ct.best2.get(), numComp);
-  hipLaunchKernelGGL(componentMinKernel, gridFor(n), dim3(256), 0, stream(), nearest, comp, orig,
-                     n, ct.best1.get());
+  if (numComp <= kLdsComponents)
+    hipLaunchKernelGGL(componentMinLdsKernel, dim3(std::max(1, std::min(1024, (n + 255) / 256))),
+                       dim3(256), 0, stream(), nearest, comp, orig, n, numComp, ct.best1.get());
+  else
+    hipLaunchKernelGGL(componentMinKernel, gridFor(n), dim3(256), 0, stream(), nearest, comp,
+                       orig, n, ct.best1.get());
   hipLaunchKernelGGL(componentMin2Kernel, gridFor(n), dim3(256), 0, stream(), nearest, comp, orig,
                      n, ct.best1.get(), ct.best2.get());
   hipLaunchKernelGGL(hookKernel, gridFor(numComp), dim3(256), 0, stream(), ct.best1.get(),

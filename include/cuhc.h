@@ -88,6 +88,9 @@ int cuhc_device_alloc(unsigned long long bytes, void **ptr);
 int cuhc_device_free(void *ptr);
 int cuhc_copy_to_device(void *dst, const void *src, unsigned long long bytes);
 int cuhc_copy_to_host(void *dst, const void *src, unsigned long long bytes);
+/* Device-to-device copy (a plain copy kernel of known size: the benchmark
+ * uses it to scale the HBM byte counters). No counterpart in the reference. */
+int cuhc_copy_device(void *dst, const void *src, unsigned long long bytes);
 
 /* Per-phase GPU timing with HIP events on the library stream (0 = off).
  * Phases: 0 FP32 distance tiles, 1 FP16 MFMA candidates, 2 FP32 recheck,
