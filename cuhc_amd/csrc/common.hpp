@@ -154,13 +154,15 @@ struct PlainWeight {
   static constexpr bool kHasValue = false;
 };
 
-struct MReachWeight {
+// kScaled: alpha != 1 (the division is a separate instantiation: a select
+// between dist and dist / alpha would evaluate the division for every pair).
+template <bool kScaled> struct MReachWeightT {
   const float *core;
   float alpha;
   using Value = float;
   DEV Value value(int p) const { return core[p]; }
   DEV float operator()(float dist, Value ci, Value cj) const {
-    return fmaxf(fmaxf(ci, cj), alpha == 1.0f ? dist : dist / alpha);
+    return fmaxf(fmaxf(ci, cj), kScaled ? dist / alpha : dist);
   }
   static constexpr bool kHasValue = true;
 };

@@ -507,7 +507,9 @@ __global__ void iotaKernel(int *out, int n) {
 template <class Metric, class F> int withWeightFor(const Metric &, WeightSpec w, F &&f) {
   switch (w.kind) {
   case kWeightPlain: return f(PlainWeight{});
-  case kWeightMReach: return f(MReachWeight{w.core, w.alpha});
+  case kWeightMReach:
+    return w.alpha == 1.0f ? f(MReachWeightT<false>{w.core, 1.0f})
+                           : f(MReachWeightT<true>{w.core, w.alpha});
   default: return kErrInvalidAlgorithm;
   }
 }
