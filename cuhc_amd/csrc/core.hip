@@ -57,21 +57,23 @@ DEV void insertSorted(float *list, int k, float v) {
   list[pos] = v;
 }
 
-// Rows are the points themselves (rowsX null: row tile blockIdx.x of X) or
-// listed points (rowsX: numRows gathered rows, rowPos their positions in X,
-// for the exact search of selected points).
+// Rows are the points themselves (rowsX null: row tile blockIdx.x of X, or
+// tile tileList[blockIdx.x] with a list) or listed points (rowsX: numRows
+// gathered rows, rowPos their positions in X, for the exact search of
+// selected points).
 template <class T, class Metric>
 __global__ void __launch_bounds__(256)
     kthNearestFusedKernel(const float *__restrict__ X, int n, int d, Metric metric, int k,
                           float *__restrict__ core, CoreBoxes boxes, int stepLimit,
                           const float *__restrict__ rowsX, int numRows,
-                          const int *__restrict__ rowPos) {
+                          const int *__restrict__ rowPos, const int *__restrict__ tileList) {
   constexpr int TR = T::TR;
   __shared__ __align__(16) TileSmem<T> sm;
   __shared__ int skip;
   extern __shared__ float lists[]; // [DIM][k], then the row tile's box (2d) if pruning
 
-  const int row0 = blockIdx.x * T::DIM;
+  const int rowTile = tileList ? tileList[blockIdx.x] : (int)blockIdx.x;
+  const int row0 = rowTile * T::DIM;
   const float *A = rowsX ? rowsX : X;
   const int nA = rowsX ? numRows : n;
   for (int i = threadIdx.x; i < T::DIM * k; i += T::THREADS)
@@ -80,8 +82,8 @@ __global__ void __launch_bounds__(256)
   float *rowBox = lists + T::DIM * k;
   if (prune)
     for (int f = threadIdx.x; f < d; f += T::THREADS) {
-      rowBox[f] = boxes.low[(size_t)blockIdx.x * d + f];
-      rowBox[d + f] = boxes.high[(size_t)blockIdx.x * d + f];
+      rowBox[f] = boxes.low[(size_t)rowTile * d + f];
+      rowBox[d + f] = boxes.high[(size_t)rowTile * d + f];
     }
   __syncthreads();
   const float slack = 1.0f - coreBoxSlack(d);
@@ -89,7 +91,7 @@ __global__ void __launch_bounds__(256)
   const int lane = threadIdx.x & 63;
   const unsigned long long groupMask = 0xffffull << (lane & 48);
   // Listed rows have no home tile: the walk starts anywhere and covers all.
-  const int homeTile = rowsX ? (int)(blockIdx.x % numTiles) : (int)blockIdx.x;
+  const int homeTile = rowsX ? (int)(blockIdx.x % numTiles) : rowTile;
 
   const int steps = min(2 * numTiles, stepLimit);
   for (int step = 0; step < steps; ++step) {
@@ -243,8 +245,9 @@ template <class F> int withMetricCore(MetricSpec m, F &&f) {
   }
 }
 
+// tileList (device, numListed entries): only those row tiles.
 int coreFused(const float *X, int n, int d, MetricSpec m, int k, float *core, bool prune,
-              int stepLimit = 0x7fffffff) {
+              int stepLimit = 0x7fffffff, const int *tileList = nullptr, int numListed = 0) {
   return withMetricCore(m, [&](auto metric) {
     using Metric = decltype(metric);
     auto run = [&](auto fill) {
@@ -265,9 +268,9 @@ int coreFused(const float *X, int n, int d, MetricSpec m, int k, float *core, bo
         smem += 2 * d * sizeof(float);
       }
       PhaseTimer timer(kPhaseCore);
-      hipLaunchKernelGGL((kthNearestFusedKernel<T, Metric>), dim3(tiles), dim3(256), smem,
-                         stream(), X, n, d, metric, k, core, boxes, stepLimit, nullptr, 0,
-                         nullptr);
+      hipLaunchKernelGGL((kthNearestFusedKernel<T, Metric>), dim3(tileList ? numListed : tiles),
+                         dim3(256), smem, stream(), X, n, d, metric, k, core, boxes, stepLimit,
+                         nullptr, 0, nullptr, tileList);
       s((int)hipGetLastError());
       s(sync());
       return s.code;
@@ -329,7 +332,7 @@ int coreForRowsFused(const float *X, int n, int d, MetricSpec m, int k, const in
       PhaseTimer timer(kPhaseCore);
       hipLaunchKernelGGL((kthNearestFusedKernel<T, Metric>), dim3(rowTiles), dim3(256), smem,
                          stream(), X, n, d, metric, k, core, CoreBoxes{}, 0x7fffffff,
-                         rowData.get(), numRows, rows);
+                         rowData.get(), numRows, rows, nullptr);
       Status st;
       st((int)hipGetLastError());
       st(sync());
@@ -365,6 +368,32 @@ int coreForRows(const float *X, int n, int d, MetricSpec m, int k, const int *ro
     s((int)hipGetLastError());
   }
   s(sync());
+  return s.code;
+}
+
+int coreForTiles(const float *X, int n, int d, MetricSpec m, int k, const int *hostTiles,
+                 int numTiles, float *core) {
+  constexpr int kDim = 128; // the fused kernel's row tile for box metrics
+  if (numTiles <= 0)
+    return 0;
+  Status s;
+  if (k <= kMaxFusedK && metricHasBoxBound(m.id)) {
+    DevArray<int> list;
+    s(list.alloc(numTiles));
+    s(upload(list, hostTiles, numTiles * sizeof(int)));
+    if (s.ok())
+      s(coreFused(X, n, d, m, k, core, true, 0x7fffffff, list, numTiles));
+    return s.code;
+  }
+  std::vector<int> rows;
+  for (int t = 0; t < numTiles; ++t)
+    for (int p = hostTiles[t] * kDim; p < std::min(n, (hostTiles[t] + 1) * kDim); ++p)
+      rows.push_back(p);
+  DevArray<int> dRows;
+  s(dRows.alloc(rows.size()));
+  s(upload(dRows, rows.data(), rows.size() * sizeof(int)));
+  if (s.ok())
+    s(coreForRows(X, n, d, m, k, dRows, (int)rows.size(), core));
   return s.code;
 }
 

@@ -34,7 +34,7 @@ namespace cuhc {
 namespace {
 
 constexpr int kCandPerRow = 512;
-constexpr int kTrainSamples = 4096;
+constexpr int kTrainSamples = 8192;
 std::atomic<int> gTrainSteps{300};
 constexpr float kTrainRate = 0.01f;
 
@@ -284,7 +284,7 @@ int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, 
   const int tiles = (n + DIM - 1) / DIM;
   Status s;
   DevArray<float> local1, local3, low, high, centroid, diag, feats, params, radius, pred, cand;
-  DevArray<int> sampleRows, count, fallback, numFallback;
+  DevArray<int> count, fallback, numFallback;
   DevArray<unsigned long long> counters;
   s(local1.alloc(n));
   s(local3.alloc(n));
@@ -312,14 +312,17 @@ int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, 
                      local1.get(), local3.get(), centroid.get(), diag.get(), feats.get());
   s((int)hipGetLastError());
   // 2. Training sample with exact core distances (written into core[], which
-  //    is overwritten below).
-  const int ms = std::min(kTrainSamples, n);
-  std::vector<int> rows(ms);
-  for (int i = 0; i < ms; ++i)
-    rows[i] = (int)(((long long)i * n) / ms);
-  s(sampleRows.alloc(ms));
-  s(upload(sampleRows, rows.data(), ms * sizeof(int)));
-  s(coreForRows(X, n, d, m, k, sampleRows, ms, core));
+  //    is overwritten below): whole row tiles spread evenly over the order, so
+  //    that the exact search is the pruned walk of those tiles.
+  const int sampleTiles = std::min(tiles, kTrainSamples / DIM);
+  std::vector<int> tileList(sampleTiles), rows;
+  for (int i = 0; i < sampleTiles; ++i) {
+    tileList[i] = (int)(((long long)i * tiles) / sampleTiles);
+    for (int p = tileList[i] * DIM; p < std::min(n, (tileList[i] + 1) * DIM); ++p)
+      rows.push_back(p);
+  }
+  const int ms = (int)rows.size();
+  s(coreForTiles(X, n, d, m, k, tileList.data(), sampleTiles, core));
   std::vector<float> hf((size_t)n * 4), hcore(n);
   s(download(hf.data(), feats, hf.size() * sizeof(float)));
   s(download(hcore.data(), core, n * sizeof(float)));

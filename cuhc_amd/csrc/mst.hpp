@@ -88,6 +88,11 @@ int coreLocal(const float *X, int n, int d, MetricSpec m, int k, int stepLimit, 
 // Exact core distances of the listed points (device positions) against all points.
 int coreForRows(const float *X, int n, int d, MetricSpec m, int k, const int *rows, int numRows,
                 float *core);
+// Exact core distances of every point of the listed 128-point row tiles
+// (host list), with the pruned walk when k <= kMaxFusedK and the metric has a
+// box bound, else through coreForRows.
+int coreForTiles(const float *X, int n, int d, MetricSpec m, int k, const int *hostTiles,
+                 int numTiles, float *core);
 int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, bool spatial);
 
 struct LearnedStats {
