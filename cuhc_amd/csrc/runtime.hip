@@ -1,6 +1,7 @@
 #include "runtime.hpp"
 
 #include <mutex>
+#include <unordered_map>
 
 namespace cuhc {
 
@@ -38,6 +39,88 @@ int numCUs() {
     gNumCUs = cus;
   }
   return gNumCUs;
+}
+
+// Device memory cache. hipMalloc and hipFree of large blocks take
+// milliseconds each (page mapping; hipFree also waits for the device), and
+// every library call allocates its working buffers afresh, so freed blocks
+// are kept per size class (the request rounded up to 2 MiB above 1 MiB, to
+// 256 B below) and handed to the next request of the same class. All work
+// runs on the one library stream, so a reused block is only touched by work
+// ordered after every earlier use of it. The cache holds at most a quarter of
+// device memory; when hipMalloc fails, the cache is released and the request
+// retried. It is never freed at exit (the HIP runtime may be gone by then).
+namespace {
+struct DeviceCache {
+  std::mutex mutex;
+  std::unordered_map<size_t, std::vector<void *>> blocks;
+  size_t bytes = 0, limit = 0;
+};
+
+DeviceCache &deviceCache() {
+  static DeviceCache *cache = new DeviceCache;
+  return *cache;
+}
+
+size_t sizeClass(size_t bytes) {
+  constexpr size_t kLarge = (size_t)2 << 20, kSmall = 256;
+  return bytes > ((size_t)1 << 20) ? (bytes + kLarge - 1) / kLarge * kLarge
+                                    : (bytes + kSmall - 1) / kSmall * kSmall;
+}
+
+void releaseCache(DeviceCache &cache) {
+  (void)hipStreamSynchronize(stream());
+  for (auto &entry : cache.blocks)
+    for (void *p : entry.second)
+      (void)hipFree(p);
+  cache.blocks.clear();
+  cache.bytes = 0;
+}
+} // namespace
+
+int deviceAlloc(void **p, size_t bytes) {
+  DeviceCache &cache = deviceCache();
+  const size_t cls = sizeClass(bytes);
+  std::lock_guard<std::mutex> lock(cache.mutex);
+  auto it = cache.blocks.find(cls);
+  if (it != cache.blocks.end() && !it->second.empty()) {
+    *p = it->second.back();
+    it->second.pop_back();
+    cache.bytes -= cls;
+    return 0;
+  }
+  hipError_t e = hipMalloc(p, cls);
+  if (e == hipErrorOutOfMemory && cache.bytes > 0) {
+    (void)hipGetLastError();
+    releaseCache(cache);
+    e = hipMalloc(p, cls);
+  }
+  return (int)e;
+}
+
+void deviceFree(void *p, size_t bytes) {
+  DeviceCache &cache = deviceCache();
+  const size_t cls = sizeClass(bytes);
+  std::lock_guard<std::mutex> lock(cache.mutex);
+  if (cache.limit == 0) {
+    size_t freeBytes = 0, totalBytes = 0;
+    if (hipMemGetInfo(&freeBytes, &totalBytes) == hipSuccess && totalBytes > 0)
+      cache.limit = totalBytes / 4;
+    else
+      cache.limit = (size_t)16 << 30;
+  }
+  if (cache.bytes + cls > cache.limit) {
+    (void)hipFree(p);
+    return;
+  }
+  cache.blocks[cls].push_back(p);
+  cache.bytes += cls;
+}
+
+size_t cachedDeviceBytes() {
+  DeviceCache &cache = deviceCache();
+  std::lock_guard<std::mutex> lock(cache.mutex);
+  return cache.bytes;
 }
 
 int upload(void *dst, const void *src, size_t bytes) {

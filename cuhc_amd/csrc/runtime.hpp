@@ -26,6 +26,13 @@ struct Status {
   bool ok() const { return code == 0; }
 };
 
+// Device memory through the library's cache of freed blocks (runtime.hip).
+// deviceFree takes the size given to deviceAlloc. cachedDeviceBytes: memory
+// held by the cache (free for the library's own use).
+int deviceAlloc(void **p, size_t bytes);
+void deviceFree(void *p, size_t bytes);
+size_t cachedDeviceBytes();
+
 // Owning device allocation.
 template <class T> struct DevArray {
   T *p = nullptr;
@@ -36,14 +43,19 @@ template <class T> struct DevArray {
   ~DevArray() { release(); }
   int alloc(size_t n) {
     release();
-    count = n;
     if (n == 0)
       return 0;
-    return (int)hipMalloc(&p, n * sizeof(T));
+    void *q = nullptr;
+    const int s = deviceAlloc(&q, n * sizeof(T));
+    if (s == 0) {
+      p = static_cast<T *>(q);
+      count = n;
+    }
+    return s;
   }
   void release() {
     if (p)
-      (void)hipFree(p);
+      deviceFree(p, count * sizeof(T));
     p = nullptr;
     count = 0;
   }

@@ -225,6 +225,7 @@ bool wardMatrixFits(int m) {
   size_t freeBytes = 0, totalBytes = 0;
   if (hipMemGetInfo(&freeBytes, &totalBytes) != hipSuccess)
     return false;
+  freeBytes += cachedDeviceBytes();
   return 4.0 * ((double)m * (m - 1) / 2) <= 0.9 * (double)freeBytes;
 }
 
