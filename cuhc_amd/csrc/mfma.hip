@@ -57,7 +57,9 @@ namespace {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kTileCols = 128;
+// Columns per staged tile: 256 for DP <= 32 (two workgroups of 256 threads
+// still fit a CU's LDS with three stages), 128 for DP = 64.
+HD constexpr int tileColsFor(int DP) { return DP <= 32 ? 256 : 128; }
 constexpr int kWaveRows = 128;     // 4 row blocks of 32
 constexpr int kBlockRows = 4 * kWaveRows;
 constexpr int kCandCap = 32;
@@ -169,14 +171,14 @@ HD int swizzleChunk(int c, int k, int chunks) {
   return k ^ ((c ^ (c >> 3)) & 7);
 }
 
-// Norm slots per column tile in normp (the tile's 128 norms, then padding, so
-// that each of the four waves stages one 256-byte slice).
+// Norm slots per column tile in normp (the tile's norms, then padding to 256,
+// so that each of the four waves stages one 256-byte slice).
 constexpr int kNormSlots = 256;
 
 // Permuted copies of the FP16 points (swizzled chunks) and norms; padding
 // columns get zero coordinates and +inf norms (v = +inf, never a candidate).
 __global__ void permuteHalfKernel(const _Float16 *xh, const float *norm, const int *perm, int n,
-                                  int nPad, int DP, _Float16 *xhp, float *normp) {
+                                  int nPad, int DP, int tileCols, _Float16 *xhp, float *normp) {
   const int vecs = DP / 8;
   const size_t total = (size_t)nPad * vecs;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
@@ -189,11 +191,13 @@ __global__ void permuteHalfKernel(const _Float16 *xh, const float *norm, const i
       for (int i = 0; i < 8; ++i)
         h[i] = (_Float16)0.0f;
     }
-    *reinterpret_cast<half8 *>(xhp + (size_t)p * DP + 8 * swizzleChunk(p & 127, v, vecs)) = h;
+    const int c = p % tileCols;
+    *reinterpret_cast<half8 *>(xhp + (size_t)p * DP + 8 * swizzleChunk(c, v, vecs)) = h;
     if (v == 0) {
-      const size_t slot = (size_t)(p >> 7) * kNormSlots + (p & 127);
+      const size_t slot = (size_t)(p / tileCols) * kNormSlots + c;
       normp[slot] = p < n ? norm[perm[p]] : INFINITY;
-      normp[slot + 128] = INFINITY;
+      if (c + tileCols < kNormSlots)
+        normp[slot + tileCols] = INFINITY;
     }
   }
 }
@@ -212,7 +216,8 @@ struct CandArgs {
   float Rmax, rhoMax;
   float gamma;  // (K + 8) 2^-23
   double kappa; // ((1 + eps)(1 + 2^-21) / (1 - eps)) with eps = (d + 4) 2^-24
-  uint2 *cand;  // (numRows x 2 halves) x kCandCap: (half-block, block minimum bits)
+  uint2 *cand;  // (numRows x 2 halves) x kCandCap: (half-block, block minimum bits); half-block
+                // b is lane half b & 1 of the 32-column block b >> 1
   int *candCount; // numRows x 2 (zeroed before a launch with column splits)
   unsigned int *theta;  // numRows, order-preserving bits (thetaBits), minimum over splits
   unsigned int *rowMin; // numRows, thetaBits of the row minimum shared by column splits
@@ -380,6 +385,7 @@ DEV void ldsDirectLoad4(const void *g, const void *lds) {
 template <int KB, bool SPLIT>
 __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
   constexpr int DP = 16 * KB;
+  constexpr int kTileCols = tileColsFor(DP);
   constexpr int TILE_BYTES = kTileCols * DP * 2;
   constexpr int LOADS = TILE_BYTES / (256 * 16); // 16-byte LDS-direct loads per thread per tile
   static_assert(TILE_BYTES % (256 * 16) == 0, "tile staging");
@@ -559,8 +565,8 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
                                       kMask ? rs.cs[wave][rb][lane] : 0,
                                       kMask ? rs.ce[wave][rb][lane] : 0);
             if (!dry)
-              recordPair(rl[rb], rowBase + rb * 32 + l32, mA[rb], mB, t * 8 + cb * 2 + hi, a,
-                         hi, shared);
+              recordPair(rl[rb], rowBase + rb * 32 + l32, mA[rb], mB,
+                         (t * (kTileCols / 32) + cb) * 2 + hi, a, hi, shared);
           }
         }
       };
@@ -720,10 +726,9 @@ __global__ void __launch_bounds__(256) recheckKernel(RecheckArgs a) {
       const int e = sub == 0 ? pick[0] : sub == 1 ? pick[1] : sub == 2 ? pick[2] : pick[3];
       const int blk = (int)__shfl(entry.x, e < 0 ? 0 : e, 64);
       if (e >= 0) {
-        // Half-block b = tile * 8 + cb * 2 + hi covers columns
-        // tile*128 + cb*32 + g*8 + hi*4 + q, g, q in 0..3.
-        const int col = (blk >> 3) * kTileCols + ((blk >> 1) & 3) * 32 + (idx >> 2) * 8 +
-                        (blk & 1) * 4 + (idx & 3);
+        // Half-block b (32-column block b >> 1, lane half b & 1) covers
+        // columns (b >> 1) * 32 + g * 8 + (b & 1) * 4 + q, g, q in 0..3.
+        const int col = (blk >> 1) * 32 + (idx >> 2) * 8 + (blk & 1) * 4 + (idx & 3);
         best = minKey(best, recheckColumn(a, xi, col, cs, ce));
       }
     }
@@ -846,7 +851,7 @@ int checkLayout() {
 bool mixedPassSupports(int d) { return d >= 1 && d <= 64; }
 
 struct MixedPass {
-  int n = 0, d = 0, DP = 0, nPad = 0;
+  int n = 0, d = 0, DP = 0, nPad = 0, tileCols = 0;
   const float *X = nullptr;
   DevArray<_Float16> xh, xhp;
   DevArray<float> norm, normp, R, rho, mu, xp;
@@ -869,7 +874,8 @@ MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
   mp->n = n;
   mp->d = d;
   mp->DP = d <= 16 ? 16 : d <= 32 ? 32 : 64; // a power of two: swizzled 16-byte chunks
-  mp->nPad = (n + kTileCols - 1) / kTileCols * kTileCols;
+  mp->tileCols = tileColsFor(mp->DP);
+  mp->nPad = (n + mp->tileCols - 1) / mp->tileCols * mp->tileCols;
   mp->X = X;
   Status s;
   DevArray<double> sums;
@@ -880,7 +886,7 @@ MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
   s(mp->xh.alloc((size_t)n * mp->DP));
   s(mp->xhp.alloc((size_t)mp->nPad * mp->DP));
   s(mp->norm.alloc(n));
-  s(mp->normp.alloc((size_t)mp->nPad / kTileCols * kNormSlots));
+  s(mp->normp.alloc((size_t)mp->nPad / mp->tileCols * kNormSlots));
   s(mp->R.alloc(n));
   s(mp->rho.alloc(n));
   s(mp->perm.alloc(n));
@@ -1016,7 +1022,7 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   if (!s.ok() || numRows == 0)
     return s.code;
   hipLaunchKernelGGL(permuteHalfKernel, dim3(2048), dim3(256), 0, stream(), mp->xh.get(),
-                     mp->norm.get(), mp->perm.get(), n, mp->nPad, mp->DP, mp->xhp.get(),
+                     mp->norm.get(), mp->perm.get(), n, mp->nPad, mp->DP, mp->tileCols, mp->xhp.get(),
                      mp->normp.get());
   s(gatherRows(mp->X, mp->perm, n, mp->d, mp->xp));
   CandArgs ca;
@@ -1044,10 +1050,10 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   ca.recorded = mp->stats.get();
   const int blocks = (numRows + kBlockRows - 1) / kBlockRows;
   // Few rows (late rounds): split the columns so that every CU gets two
-  // workgroups; each split keeps at least 16 tiles.
+  // workgroups; each split keeps at least 2,048 columns.
   const int target = 2 * numCUs();
   int splits = blocks >= target ? 1 : (target + blocks - 1) / blocks;
-  splits = std::max(1, std::min({splits, 8, mp->nPad / kTileCols / 16}));
+  splits = std::max(1, std::min({splits, 8, mp->nPad / 2048}));
   s(fill(mp->theta, 0xff, (size_t)numRows * sizeof(unsigned int)));
   if (splits > 1) {
     s(fill(mp->candCount, 0, (size_t)numRows * 2 * sizeof(int)));
