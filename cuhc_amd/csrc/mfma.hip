@@ -559,13 +559,21 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
 #pragma unroll
           for (int rb = 0; rb < 4; ++rb)
             acc[rb] = mfmaBlock<KB>(o1, bop[rb]);
+          float mB[4];
+          bool record = false;
 #pragma unroll
           for (int rb = 0; rb < 4; ++rb) {
-            const float mB = blockMin(acc[rb], rl[rb], kMask, c0 + (cb + 1) * 32, lane,
-                                      kMask ? rs.cs[wave][rb][lane] : 0,
-                                      kMask ? rs.ce[wave][rb][lane] : 0);
-            if (!dry)
-              recordPair(rl[rb], rowBase + rb * 32 + l32, mA[rb], mB,
+            mB[rb] = blockMin(acc[rb], rl[rb], kMask, c0 + (cb + 1) * 32, lane,
+                              kMask ? rs.cs[wave][rb][lane] : 0,
+                              kMask ? rs.ce[wave][rb][lane] : 0);
+            record |= fminf(mA[rb], mB[rb]) <= rl[rb].th;
+          }
+          // One wave-uniform branch for the four row blocks: in the common
+          // case no lane records anything.
+          if (!dry && __ballot(record) != 0) {
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+              recordPair(rl[rb], rowBase + rb * 32 + l32, mA[rb], mB[rb],
                          (t * (kTileCols / 32) + cb) * 2 + hi, a, hi, shared);
           }
         }
