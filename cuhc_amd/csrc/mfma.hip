@@ -1057,11 +1057,25 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   ca.rowMin = mp->rowMin;
   ca.recorded = mp->stats.get();
   const int blocks = (numRows + kBlockRows - 1) / kBlockRows;
-  // Few rows (late rounds): split the columns so that every CU gets two
-  // workgroups; each split keeps at least 2,048 columns.
-  const int target = 2 * numCUs();
-  int splits = blocks >= target ? 1 : (target + blocks - 1) / blocks;
-  splits = std::max(1, std::min({splits, 8, mp->nPad / 2048}));
+  // Column splits (each keeps at least 2,048 columns): the fewest splits
+  // whose workgroups fill the last wave of resident slots (two per CU) at
+  // least 90%, else the best fill. Late rounds with few rows would otherwise
+  // leave CUs idle, and a grid just above a multiple of the resident slots
+  // would run its last wave nearly empty.
+  const long long slots = 2LL * numCUs();
+  const int maxSplits = std::max(1, std::min(8, mp->nPad / 2048));
+  int splits = 1;
+  double bestFill = 0.0;
+  for (int sp = 1; sp <= maxSplits; ++sp) {
+    const long long units = (long long)blocks * sp;
+    const double fill = (double)units / (double)(((units + slots - 1) / slots) * slots);
+    if (fill > bestFill + 1e-9) {
+      bestFill = fill;
+      splits = sp;
+    }
+    if (fill >= 0.9)
+      break;
+  }
   s(fill(mp->theta, 0xff, (size_t)numRows * sizeof(unsigned int)));
   if (splits > 1) {
     s(fill(mp->candCount, 0, (size_t)numRows * 2 * sizeof(int)));
