@@ -15,13 +15,14 @@
 // distance r = |x'_i - x'_j| = s |x_i - x_j| satisfies | r - |xh_i - xh_j| |
 // <= delta_i = rho_i + rho_max. The FP32 direct-form value sigma of the pair
 // (the fma fold) is within (d + 4) 2^-24 of the exact squared distance.
-// Per row, from the running minimum m of v over valid columns:
-//   A     = sqrt(m + N_i + E_i) + delta_i            >= r of the best pair
+// Per row, from m = the kTopK-th smallest half-block minimum of v seen so far
+// (kTopK distinct columns have v <= m):
+//   A     = sqrt(m + N_i + E_i) + delta_i            >= r of those columns
 //   theta = (A sqrt(kappa) + delta_i)^2 + E_i - N_i
 // with kappa covering both FP32 roundings and a 2^-21 margin for ties of
 // sqrtf. A column with v_ij > theta has a larger FP32 value than the row's
-// best (or than its sqrt ties), so it cannot be the row's argmin; every
-// column with v_ij <= theta is a candidate.
+// kTopK-th nearest (or than its sqrt ties), so it is not among the row's
+// kTopK nearest; every column with v_ij <= theta is a candidate.
 //
 // Rows are the points still to be searched this round (in component order);
 // columns are all points permuted so that every component is a contiguous
@@ -35,8 +36,13 @@
 // in its own list when the minimum is within its theta (the two halves of a
 // row share their minimum once per tile, so theta stays tight). The recheck
 // kernel recomputes the FP32 direct-form weights of the recorded half-blocks
-// still within the row's final theta and keeps the exact argmin (weight,
-// original index). Rows whose list overflows are rechecked in full. This
+// still within the row's final theta and keeps the exact kTopK smallest keys
+// (weight, original index). The next round takes a row's nearest point in
+// another component from that list when one of its entries has left the
+// row's component (the entries were the nearest outside a subset of the new
+// component, so the first such entry is exact); only rows whose whole list
+// joined their component are searched again. Rows whose list overflows are
+// rechecked in full. This
 // translation unit is compiled with -fno-honor-nans (inputs are validated
 // finite), which lets fminf map to v_min/v_min3 without canonicalisation.
 #include "mst.hpp"
@@ -62,7 +68,8 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 HD constexpr int tileColsFor(int DP) { return DP <= 32 ? 256 : 128; }
 constexpr int kWaveRows = 128;     // 4 row blocks of 32
 constexpr int kBlockRows = 4 * kWaveRows;
-constexpr int kCandCap = 32;
+constexpr int kCandCap = 64; // list entries per (row, half)
+constexpr int kTopK = 4; // nearest points kept per row for the next round
 
 std::atomic<int> gLayoutChecked{0}; // 0 unknown, 1 ok, -1 mismatch
 unsigned long long gStats[4];
@@ -222,6 +229,7 @@ struct CandArgs {
   unsigned int *theta;  // numRows, order-preserving bits (thetaBits), minimum over splits
   unsigned int *rowMin; // numRows, thetaBits of the row minimum shared by column splits
   unsigned long long *recorded; // [0] entries recorded, [4] row-column pairs computed
+  int rank; // theta from the rank-th smallest half-block minimum (1 or kTopK)
 };
 
 // Order-preserving 32-bit image of a float (any sign), for atomicMin across
@@ -274,7 +282,7 @@ DEV void loadBlock(BlockOps<KB> &o, const _Float16 *sx, const float *sn, int cb,
 // half of the row's columns). The rest of the row's state (component range,
 // norm and bounds, row minimum) is read rarely and lives in LDS (RowShared).
 struct RowLane {
-  float runMin; // this lane's running minimum of v
+  float top[kTopK]; // ascending: this lane's smallest half-block minima (see tile end)
   float th;     // theta of the row minimum at the last tile end
   int count;    // entries recorded in this lane's list
 };
@@ -299,9 +307,8 @@ DEV f32x16 mfmaBlock(const BlockOps<KB> &o, const half8 (&bop)[KB]) {
 }
 
 // This lane's minimum over its 16 values of one block (same-component columns
-// masked to +inf when the tile crosses the row's component), folded into its
-// running minimum.
-DEV float blockMin(f32x16 acc, RowLane &rl, bool anyMask, int j0, int lane, int cs, int ce) {
+// masked to +inf when the tile crosses the row's component).
+DEV float blockMin(f32x16 acc, bool anyMask, int j0, int lane, int cs, int ce) {
   if (anyMask) {
     // Column of register r: j0 + mOfReg(r, lane) = x + (r / 4) * 8 + r % 4.
     const unsigned span = (unsigned)(ce - cs);
@@ -317,9 +324,15 @@ DEV float blockMin(f32x16 acc, RowLane &rl, bool anyMask, int j0, int lane, int 
   const float m2 = fminf(fminf(acc[6], acc[7]), acc[8]);
   const float m3 = fminf(fminf(acc[9], acc[10]), acc[11]);
   const float m4 = fminf(fminf(acc[12], acc[13]), acc[14]);
-  const float m = fminf(fminf(fminf(m0, m1), m2), fminf(fminf(m3, m4), acc[15]));
-  rl.runMin = fminf(rl.runMin, m);
-  return m;
+  return fminf(fminf(fminf(m0, m1), m2), fminf(fminf(m3, m4), acc[15]));
+}
+
+// Sorted insertion into the lane's kTopK smallest half-block minima.
+DEV void insertTop(RowLane &rl, float m) {
+#pragma unroll
+  for (int q = kTopK - 1; q > 0; --q)
+    rl.top[q] = fminf(rl.top[q], fmaxf(rl.top[q - 1], m));
+  rl.top[0] = fminf(rl.top[0], m);
 }
 
 // One list per (row, half). With column splits several workgroups append to
@@ -336,12 +349,15 @@ DEV void recordBlock(RowLane &rl, int slot, float m, int block, const CandArgs &
 
 // Records this lane's 16 columns of each of two blocks when they can hold a
 // candidate (the lane's theta is never below the row's final one, so nothing
-// needed is missed). th is -inf for rows outside the range and FLT_MAX before
-// the first tile end, so one comparison also drops wholly masked blocks. One
-// branch per pair of blocks in the common case that neither records.
+// needed is missed), and inserts the two minima into the lane's smallest
+// ones. th is -inf for rows outside the range, so one comparison also drops
+// wholly masked blocks. A minimum above th is never among the row's kTopK
+// smallest, so skipping its insertion only keeps theta conservative.
 DEV void recordPair(RowLane &rl, int slot, float mA, float mB, int blockA, const CandArgs &a,
                     int hi, bool shared) {
   if (fminf(mA, mB) <= rl.th) {
+    insertTop(rl, mA);
+    insertTop(rl, mB);
     if (mA <= rl.th)
       recordBlock(rl, slot, mA, blockA, a, hi, shared);
     if (mB <= rl.th)
@@ -437,7 +453,9 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
     rs.E[wave][rb][lane] = a.gamma * Rs * Rs * (1.0f + 1e-6f);
     rs.D[wave][rb][lane] = (valid ? a.rho[orig] : 0.0f) + a.rhoMax;
     rs.rowMin[wave][rb][lane] = INFINITY;
-    rl[rb].runMin = INFINITY;
+#pragma unroll
+    for (int q = 0; q < kTopK; ++q)
+      rl[rb].top[q] = INFINITY;
     rl[rb].th = valid ? FLT_MAX : -INFINITY;
     rl[rb].count = 0;
   }
@@ -551,7 +569,7 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
           float mA[4];
 #pragma unroll
           for (int rb = 0; rb < 4; ++rb)
-            mA[rb] = blockMin(acc[rb], rl[rb], kMask, c0 + cb * 32, lane,
+            mA[rb] = blockMin(acc[rb], kMask, c0 + cb * 32, lane,
                               kMask ? rs.cs[wave][rb][lane] : 0,
                               kMask ? rs.ce[wave][rb][lane] : 0);
           if (cb + 2 < kTileCols / 32)
@@ -563,14 +581,21 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
           bool record = false;
 #pragma unroll
           for (int rb = 0; rb < 4; ++rb) {
-            mB[rb] = blockMin(acc[rb], rl[rb], kMask, c0 + (cb + 1) * 32, lane,
+            mB[rb] = blockMin(acc[rb], kMask, c0 + (cb + 1) * 32, lane,
                               kMask ? rs.cs[wave][rb][lane] : 0,
                               kMask ? rs.ce[wave][rb][lane] : 0);
             record |= fminf(mA[rb], mB[rb]) <= rl[rb].th;
           }
           // One wave-uniform branch for the four row blocks: in the common
-          // case no lane records anything.
-          if (!dry && __ballot(record) != 0) {
+          // case no lane records anything. The dry run records nothing and
+          // keeps every minimum.
+          if (dry) {
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+              insertTop(rl[rb], mA[rb]);
+              insertTop(rl[rb], mB[rb]);
+            }
+          } else if (__ballot(record) != 0) {
 #pragma unroll
             for (int rb = 0; rb < 4; ++rb)
               recordPair(rl[rb], rowBase + rb * 32 + l32, mA[rb], mB[rb],
@@ -582,11 +607,32 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
         runTile(std::true_type{});
       else
         runTile(std::false_type{});
-      // Tile end: both halves of each row agree on the row minimum and theta.
+      // Tile end: both halves of each row agree on the row's kTopK smallest
+      // half-block minima and theta. The merged list stays with the lane of
+      // half 0 and half 1 starts empty, so no minimum is counted twice; after
+      // the dry run both start empty (its tile is visited again).
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) {
-        float m = fminf(rl[rb].runMin, __shfl_xor(rl[rb].runMin, 32, 64));
-        rl[rb].runMin = m;
+        float c[kTopK];
+#pragma unroll
+        for (int q = 0; q < kTopK; ++q)
+          c[q] = fminf(rl[rb].top[q], __shfl_xor(rl[rb].top[kTopK - 1 - q], 32, 64));
+        // c is bitonic (ascending against descending); sort it.
+        static_assert(kTopK == 4, "bitonic merge of four");
+        auto cx = [](float &x, float &y) {
+          const float lo = fminf(x, y), hi2 = fmaxf(x, y);
+          x = lo;
+          y = hi2;
+        };
+        cx(c[0], c[2]);
+        cx(c[1], c[3]);
+        cx(c[0], c[1]);
+        cx(c[2], c[3]);
+        const bool keep = !dry && hi == 0;
+#pragma unroll
+        for (int q = 0; q < kTopK; ++q)
+          rl[rb].top[q] = keep ? c[q] : INFINITY;
+        float m = a.rank == kTopK ? c[kTopK - 1] : c[0];
         const bool improved = m < rs.rowMin[wave][rb][lane] && rs.cs[wave][rb][lane] >= 0;
         if (shared) {
           // Publish an improvement; take the best any split has published.
@@ -627,11 +673,11 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Exact FP32 recheck: one wave per row. The row's 64 list entries (32 per
-// half) are loaded at once, one per lane; the entries still within the row's
-// final theta are taken four at a time, 16 lanes per half-block, one column per
-// lane. Column points come from the permuted FP32 copy xp (the same values as
-// X), so no load waits on the permutation.
+// Exact FP32 recheck: one wave per row. The row's list entries (kCandCap per
+// half) are read 32 per half at a time, one per lane; the entries still within
+// the row's final theta are taken four at a time, 16 lanes per half-block, one
+// column per lane. Column points come from the permuted FP32 copy xp (the same
+// values as X), so no load waits on the permutation.
 
 struct RecheckArgs {
   const float *X;  // n x d, original order
@@ -645,6 +691,8 @@ struct RecheckArgs {
   const unsigned int *theta; // thetaBits
   bool sqeuclidean;
   unsigned long long *nearest; // by original index
+  unsigned long long *topk;    // by original index: kTopK ascending keys (kEmptyKey padded)
+  int rank;                    // keys certified per row (the candidate pass's rank)
   // kStatStripes stripes of one 128-byte line each: [1] half-blocks rechecked,
   // [2] pairs, [3] overflow rows (striped so that a million waves do not
   // serialise on one address).
@@ -694,58 +742,99 @@ DEV unsigned long long recheckColumn(const RecheckArgs &a, const float *xi, int 
 }
 
 __global__ void __launch_bounds__(256) recheckKernel(RecheckArgs a) {
-  static_assert(kCandCap == 32, "one list entry per lane");
+  static_assert(kCandCap % 32 == 0, "lists are read 32 entries per half at a time");
   __shared__ float rowX[4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = blockIdx.x * 4 + wave;
   if (r >= a.numRows || a.rows[3 * r] < 0)
     return; // wave-uniform (padding slot or past the end); no block-wide barrier below
-  // Independent loads: the row, its counts, theta and every list slot (slots
-  // past a count hold stale data and are masked below).
+  // Independent loads: the row, its counts, theta and the first 32 list slots
+  // of each half (slots past a count hold stale data and are masked below).
   const int pos = a.rows[3 * r], cs = a.rows[3 * r + 1], ce = a.rows[3 * r + 2];
   const int c0 = a.candCount[(size_t)r * 2], c1 = a.candCount[(size_t)r * 2 + 1];
   const float th = thetaFromBits(a.theta[r]);
-  const uint2 entry = a.cand[(size_t)r * 2 * kCandCap + lane];
+  const int half = lane >> 5, e32 = lane & 31;
+  const size_t listBase = ((size_t)r * 2 + half) * kCandCap;
+  uint2 entry = a.cand[listBase + e32];
   float *xi = rowX[wave];
   for (int k = lane; k < a.d; k += 64)
     xi[k] = a.xp[(size_t)pos * a.d + k];
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  unsigned long long best = kEmptyKey;
+  // This lane's kTopK smallest keys (ascending), merged over the wave below.
+  unsigned long long best[kTopK];
+#pragma unroll
+  for (int q = 0; q < kTopK; ++q)
+    best[q] = kEmptyKey;
+  auto insert = [&](unsigned long long key) {
+#pragma unroll
+    for (int q = kTopK - 1; q > 0; --q)
+      best[q] = key < best[q] ? (key > best[q - 1] ? key : best[q - 1]) : best[q];
+    best[0] = minKey(best[0], key);
+  };
   unsigned long long blocks = 0;
   const bool overflow = c0 > kCandCap || c1 > kCandCap;
   if (overflow) {
     for (int col = lane; col < a.n; col += 64)
-      best = minKey(best, recheckColumn(a, xi, col, cs, ce));
+      insert(recheckColumn(a, xi, col, cs, ce));
   } else {
-    const bool listed = lane < 32 ? lane < c0 : lane - 32 < c1;
-    unsigned long long live = __ballot(listed && __uint_as_float(entry.y) <= th);
-    blocks = (unsigned long long)__popcll(live);
     const int sub = lane >> 4, idx = lane & 15;
-    while (live) {
-      // The next four live entries (wave-uniform); this lane's group takes one.
-      int pick[4];
+    const int count = half ? c1 : c0, longest = max(c0, c1);
+    // 32 entries of each half at a time.
+#pragma unroll 1
+    for (int w0 = 0; w0 < longest; w0 += 32) {
+      if (w0 > 0)
+        entry = a.cand[listBase + w0 + e32];
+      const bool listed = w0 + e32 < count;
+      unsigned long long live = __ballot(listed && __uint_as_float(entry.y) <= th);
+      blocks += (unsigned long long)__popcll(live);
+      while (live) {
+        // The next four live entries (wave-uniform); this lane's group takes one.
+        int pick[4];
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        pick[g] = live ? __ffsll((long long)live) - 1 : -1;
-        live &= live ? live - 1 : 0ull;
-      }
-      const int e = sub == 0 ? pick[0] : sub == 1 ? pick[1] : sub == 2 ? pick[2] : pick[3];
-      const int blk = (int)__shfl(entry.x, e < 0 ? 0 : e, 64);
-      if (e >= 0) {
-        // Half-block b (32-column block b >> 1, lane half b & 1) covers
-        // columns (b >> 1) * 32 + g * 8 + (b & 1) * 4 + q, g, q in 0..3.
-        const int col = (blk >> 1) * 32 + (idx >> 2) * 8 + (blk & 1) * 4 + (idx & 3);
-        best = minKey(best, recheckColumn(a, xi, col, cs, ce));
+        for (int g = 0; g < 4; ++g) {
+          pick[g] = live ? __ffsll((long long)live) - 1 : -1;
+          live &= live ? live - 1 : 0ull;
+        }
+        const int e = sub == 0 ? pick[0] : sub == 1 ? pick[1] : sub == 2 ? pick[2] : pick[3];
+        const int blk = (int)__shfl(entry.x, e < 0 ? 0 : e, 64);
+        if (e >= 0) {
+          // Half-block b (32-column block b >> 1, lane half b & 1) covers
+          // columns (b >> 1) * 32 + g * 8 + (b & 1) * 4 + q, g, q in 0..3.
+          const int col = (blk >> 1) * 32 + (idx >> 2) * 8 + (blk & 1) * 4 + (idx & 3);
+          insert(recheckColumn(a, xi, col, cs, ce));
+        }
       }
     }
   }
+  // The wave's kTopK smallest: repeatedly the minimum of the lanes' heads,
+  // popped from the lane that holds it (keys are distinct: distinct indices).
+  unsigned long long top[kTopK];
 #pragma unroll
-  for (int m = 32; m > 0; m >>= 1)
-    best = minKey(best, shflXorKey(best, m));
+  for (int q = 0; q < kTopK; ++q) {
+    unsigned long long m = best[0];
+#pragma unroll
+    for (int s2 = 32; s2 > 0; s2 >>= 1)
+      m = minKey(m, shflXorKey(m, s2));
+    top[q] = m;
+    if (m != kEmptyKey && best[0] == m) {
+#pragma unroll
+      for (int r2 = 0; r2 + 1 < kTopK; ++r2)
+        best[r2] = best[r2 + 1];
+      best[kTopK - 1] = kEmptyKey;
+    }
+  }
+  // Only the first rank keys are certified (every column that could precede
+  // them was a candidate).
+  if (lane < kTopK)
+    a.topk[(size_t)a.perm[pos] * kTopK + lane] = lane >= a.rank ? kEmptyKey
+                                                 : lane == 0   ? top[0]
+                                                 : lane == 1   ? top[1]
+                                                 : lane == 2   ? top[2]
+                                                               : top[3];
   if (lane == 0) {
-    a.nearest[a.perm[pos]] = best;
+    a.nearest[a.perm[pos]] = top[0];
     unsigned long long *st = a.stats + (size_t)(r % kStatStripes) * kStripeWords;
     if (overflow) {
       atomicAdd(&st[3], 1ull);
@@ -778,9 +867,14 @@ __global__ void componentRangesKernel(const int *__restrict__ sortedComp, int n,
     end[c] = i + 1;
 }
 
+// A row is settled when an entry of its kept list (the kTopK nearest outside
+// its component when it was last searched) lies in another component now: the
+// first such entry is its nearest there, written to nearest[]. Every other
+// row is searched again.
 __global__ void staleRowsKernel(const int *__restrict__ perm, const int *__restrict__ comp,
-                                const unsigned long long *__restrict__ prev, int n,
-                                bool firstRound, int *flag) {
+                                const unsigned long long *__restrict__ topk,
+                                unsigned long long *__restrict__ nearest, int n, bool firstRound,
+                                int *flag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n)
     return;
@@ -791,8 +885,17 @@ __global__ void staleRowsKernel(const int *__restrict__ perm, const int *__restr
   const int p = perm[i];
   bool stale = true;
   if (!firstRound) {
-    const unsigned long long key = prev[p];
-    stale = key == kEmptyKey || comp[(int)keyIndex(key)] == comp[p];
+    const int own = comp[p];
+#pragma unroll
+    for (int q = 0; q < kTopK && stale; ++q) {
+      const unsigned long long key = topk[(size_t)p * kTopK + q];
+      if (key == kEmptyKey)
+        break;
+      if (comp[(int)keyIndex(key)] != own) {
+        nearest[p] = key;
+        stale = false;
+      }
+    }
   }
   flag[i] = stale ? 1 : 0;
 }
@@ -870,7 +973,7 @@ struct MixedPass {
   DevArray<unsigned char> sortTemp, scanTemp;
   size_t sortBytes = 0, scanBytes = 0;
   DevArray<uint2> cand;
-  DevArray<unsigned long long> stats, recheckStats;
+  DevArray<unsigned long long> stats, recheckStats, topk;
   float Rmax = 0.0f, rhoMax = 0.0f;
 };
 
@@ -905,6 +1008,7 @@ MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
   s(mp->rowMin.alloc((size_t)2 * n));
   s(mp->xp.alloc((size_t)n * d));
   s(mp->cand.alloc((size_t)2 * 2 * n * kCandCap));
+  s(mp->topk.alloc((size_t)n * kTopK));
   s(mp->stats.alloc(5));
   s(mp->recheckStats.alloc((size_t)kStatStripes * kStripeWords));
   s(mp->positions.alloc(n));
@@ -1001,10 +1105,12 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   const dim3 grid((n + 255) / 256);
   hipLaunchKernelGGL(componentRangesKernel, grid, dim3(256), 0, stream(), mp->sortedComp.get(), n,
                      mp->compStart.get(), mp->compEnd.get());
-  // Rows to search: all in the first round; later only points whose previous
-  // nearest joined their own component. Listed in component order.
+  // Rows to search: all in the first round; later only points whose kept
+  // list has no entry outside their component (the others are settled here).
+  // Listed in component order.
   hipLaunchKernelGGL(staleRowsKernel, dim3((n + 1 + 255) / 256), dim3(256), 0, stream(),
-                     mp->perm.get(), comp, nearest, n, firstRound, mp->rowFlag.get());
+                     mp->perm.get(), comp, mp->topk.get(), nearest, n, firstRound,
+                     mp->rowFlag.get());
   bytes = mp->scanBytes;
   s((int)hipcub::DeviceScan::ExclusiveSum(mp->scanTemp.get(), bytes, mp->rowFlag.get(),
                                           mp->rowIndex.get(), n + 1, stream()));
@@ -1056,6 +1162,9 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   ca.theta = mp->theta;
   ca.rowMin = mp->rowMin;
   ca.recorded = mp->stats.get();
+  // The first round keeps each point's kTopK nearest, which settle most rows
+  // of the second round; later rounds keep the nearest only.
+  ca.rank = firstRound ? kTopK : 1;
   const int blocks = (numRows + kBlockRows - 1) / kBlockRows;
   // Column splits (each keeps at least 2,048 columns): the fewest splits
   // whose workgroups fill the last wave of resident slots (two per CU) at
@@ -1109,6 +1218,8 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   ra.theta = mp->theta;
   ra.sqeuclidean = sqeuclidean;
   ra.nearest = nearest;
+  ra.topk = mp->topk;
+  ra.rank = ca.rank;
   ra.stats = mp->recheckStats;
   {
     PhaseTimer timer(kPhaseRecheck);
