@@ -1172,7 +1172,7 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   // leave CUs idle, and a grid just above a multiple of the resident slots
   // would run its last wave nearly empty.
   const long long slots = 2LL * numCUs();
-  const int maxSplits = std::max(1, std::min(8, mp->nPad / 2048));
+  const int maxSplits = std::max(1, std::min(16, mp->nPad / 2048));
   int splits = 1;
   double bestFill = 0.0;
   for (int sp = 1; sp <= maxSplits; ++sp) {
