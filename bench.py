@@ -1,8 +1,8 @@
 """Benchmark: seconds per full single-linkage tree of 1,000,000 points in 32-d.
 
-Workload (the 1M-point configuration the headline metric is quoted on that is
-GEMM-shaped): n = 1,000,000 points, d = 32, float32 i.i.d.
-N(0,1) from numpy's default_rng(3); exact single linkage through Boruvka's MST.
+Workload (the 1M-point single-linkage configuration of the headline metric,
+the GEMM-shaped one): n = 1,000,000 points, d = 32, float32 i.i.d. N(0,1) from
+numpy's default_rng(3); exact single linkage through Boruvka's MST.
 A step is one full tree (scipy-format Z back in host memory) from points
 already resident in HBM (cuhc_linkage_device). The timed form is the mixed
 one: FP16 MFMA candidate tiles + certified FP32 recheck; the plain FP32 form is
@@ -193,7 +193,7 @@ def main():
                         help="only run a device-to-device copy of this size (counter scaling)")
     args = parser.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only (this build has no multi-GPU path); "
+        print("bench.py runs on one GPU only (there is no multi-GPU path); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
 

@@ -43,7 +43,7 @@ _declare("cuhc_set_pruning", _int, _int)
 _declare("cuhc_prune_stats", None, _u64p, _u64p)
 _declare("cuhc_error_string", ctypes.c_char_p, _int)
 
-# Extensions of this build.
+# Extensions (no counterpart in the reference library).
 _declare("cuhc_set_precision", _int, _int)
 _declare("cuhc_linkage_device", _int, ctypes.c_void_p, _int, _int, _int, _float, _int, _int,
          _f64p)
