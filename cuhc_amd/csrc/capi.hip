@@ -38,7 +38,8 @@ bool useMixed(int metricId, int d) {
 }
 
 // Decides pruning for points in spatial order (auto mode samples a pruned
-// pass; reference src/capi.cu:101-119).
+// pass, reference src/capi.cu:101-119; here the sampled row tiles compute
+// their first 64 walk steps and count the rest against the bound reached).
 int decidePruning(const DevicePoints &pts, int n, int d, MetricSpec m, bool &prune) {
   if (!prune || gPruneMode.load() == kPruneAlways)
     return 0;

@@ -294,7 +294,8 @@ __global__ void __launch_bounds__(256)
                            const int *__restrict__ orig, Boxes boxes,
                            unsigned int *__restrict__ compBest,
                            unsigned long long *__restrict__ nearest,
-                           unsigned long long *tilesComputed, int rowTileStride) {
+                           unsigned long long *tilesComputed, int rowTileStride,
+                           int computeSteps) {
   constexpr int TR = T::TR;
   constexpr int kBatch = 64;
   __shared__ __align__(16) TileSmem<T> sm;
@@ -397,6 +398,8 @@ __global__ void __launch_bounds__(256)
         continue; // uniform: lowerBound and bound are shared
       const int colTile = walkTile(rowTile, base + j, numTiles);
       ++computed;
+      if (base + j >= computeSteps)
+        continue; // estimate: counted against the bound reached so far, not computed
       const int col0 = colTile * T::DIM;
       float w[TR][TR];
       computeTile<T>(sm, X, n, X, n, d, metric, row0, col0, w);
@@ -700,6 +703,11 @@ struct DeviceBoxes {
   }
 };
 
+// Walk steps computed per row tile by the sampled estimate (stride > 1); later
+// steps are only counted against the bound the computed ones reached (an
+// overestimate of the pruned share: the bound would keep shrinking).
+constexpr int kEstimateSteps = 64;
+
 template <class T, class Metric, class Weight>
 int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &weight,
             const int *comp, const int *orig, unsigned int *compBest,
@@ -721,7 +729,8 @@ int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &we
     PhaseTimer timer(kPhaseDistanceTiles);
     hipLaunchKernelGGL((nearestOtherWalkKernel<T, Metric, Weight>), dim3(sampled), dim3(256),
                        2 * d * sizeof(float), stream(), X, n, d, metric, weight, comp, orig,
-                       boxes.view, compBest, nearest, counter.get(), stride);
+                       boxes.view, compBest, nearest, counter.get(), stride,
+                       stride > 1 ? kEstimateSteps : 0x7fffffff);
     s((int)hipGetLastError());
   }
   unsigned long long computed = 0;
