@@ -354,13 +354,13 @@ DEV void recordBlock(RowLane &rl, int slot, float m, int block, const CandArgs &
 // wholly masked blocks. A minimum above th is never among the row's kTopK
 // smallest, so skipping its insertion only keeps theta conservative.
 DEV void recordPair(RowLane &rl, int slot, float mA, float mB, int blockA, const CandArgs &a,
-                    int hi, bool shared) {
+                    int hi, bool shared, bool dry) {
   if (fminf(mA, mB) <= rl.th) {
     insertTop(rl, mA);
     insertTop(rl, mB);
-    if (mA <= rl.th)
+    if (!dry && mA <= rl.th)
       recordBlock(rl, slot, mA, blockA, a, hi, shared);
-    if (mB <= rl.th)
+    if (!dry && mB <= rl.th)
       recordBlock(rl, slot, mB, blockA + 2, a, hi, shared);
   }
 }
@@ -587,19 +587,16 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
             record |= fminf(mA[rb], mB[rb]) <= rl[rb].th;
           }
           // One wave-uniform branch for the four row blocks: in the common
-          // case no lane records anything. The dry run records nothing and
-          // keeps every minimum.
-          if (dry) {
-#pragma unroll
-            for (int rb = 0; rb < 4; ++rb) {
-              insertTop(rl[rb], mA[rb]);
-              insertTop(rl[rb], mB[rb]);
-            }
-          } else if (__ballot(record) != 0) {
+          // case no lane records anything. In the dry run every valid row's
+          // theta is FLT_MAX, so every finite minimum is kept and nothing is
+          // recorded. (The minima are updated in this one branch only: a
+          // second update path would make the compiler copy them between
+          // registers every iteration.)
+          if (__ballot(record) != 0) {
 #pragma unroll
             for (int rb = 0; rb < 4; ++rb)
               recordPair(rl[rb], rowBase + rb * 32 + l32, mA[rb], mB[rb],
-                         (t * (kTileCols / 32) + cb) * 2 + hi, a, hi, shared);
+                         (t * (kTileCols / 32) + cb) * 2 + hi, a, hi, shared, dry);
           }
         }
       };
