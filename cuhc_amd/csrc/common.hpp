@@ -12,11 +12,16 @@
 // Every fold is a no-op for a zero feature pair, so tiles pad features with 0.
 // The library is built with -ffp-contract=off: the only fused operations are
 // the explicit fmaf calls, and division and sqrt are correctly rounded.
+//
+// Argmin filters: for metrics with kAccFilter, accBound(x) is an upper bound
+// on every accumulator whose finish() is <= x, so an argmin pass can compare
+// the raw accumulator with it and finish only the pairs that pass.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <math.h>
+#include <float.h>
 
 #define HD __host__ __device__ __forceinline__
 #define DEV __device__ __forceinline__
@@ -49,6 +54,8 @@ struct SqEuclidean {
   using Acc = float;
   static constexpr bool kBoxBound = true;
   static constexpr bool kWide = false;
+  static constexpr bool kAccFilter = true;
+  HD float accBound(float x) const { return x; }
   HD Acc init() const { return 0.0f; }
   HD Acc fold(Acc acc, float a, float b) const {
     const float diff = a - b;
@@ -59,12 +66,17 @@ struct SqEuclidean {
 
 struct Euclidean : SqEuclidean {
   HD float finish(Acc acc) const { return sqrtf(acc); }
+  // sqrtf(a) <= x implies a <= x^2 (1 + 2^-23); the float product and the
+  // factor 1 + 2^-20 cover that and their own rounding, FLT_MIN underflow.
+  HD float accBound(float x) const { return x * x * (1.0f + 0x1p-20f) + FLT_MIN; }
 };
 
 struct Manhattan {
   using Acc = float;
   static constexpr bool kBoxBound = true;
   static constexpr bool kWide = false;
+  static constexpr bool kAccFilter = true;
+  HD float accBound(float x) const { return x; }
   HD Acc init() const { return 0.0f; }
   HD Acc fold(Acc acc, float a, float b) const { return acc + fabsf(a - b); }
   HD float finish(Acc acc) const { return acc; }
@@ -74,6 +86,8 @@ struct Chebyshev {
   using Acc = float;
   static constexpr bool kBoxBound = true;
   static constexpr bool kWide = false;
+  static constexpr bool kAccFilter = true;
+  HD float accBound(float x) const { return x; }
   HD Acc init() const { return 0.0f; }
   HD Acc fold(Acc acc, float a, float b) const { return fmaxf(acc, fabsf(a - b)); }
   HD float finish(Acc acc) const { return acc; }
@@ -94,6 +108,8 @@ struct Minkowski {
   using Acc = float;
   static constexpr bool kBoxBound = true;
   static constexpr bool kWide = false;
+  static constexpr bool kAccFilter = false;
+  HD float accBound(float) const { return INFINITY; }
   HD Acc init() const { return 0.0f; }
   HD Acc fold(Acc acc, float a, float b) const {
     const float x = fabsf(a - b);
@@ -110,6 +126,8 @@ struct Cosine {
   using Acc = CosAcc;
   static constexpr bool kBoxBound = false;
   static constexpr bool kWide = true;
+  static constexpr bool kAccFilter = false;
+  HD float accBound(float) const { return INFINITY; }
   HD Acc init() const { return CosAcc{0.0f, 0.0f, 0.0f}; }
   HD Acc fold(Acc acc, float a, float b) const {
     acc.ab = fmaf(a, b, acc.ab);
@@ -145,12 +163,14 @@ HD unsigned long long minKey(unsigned long long a, unsigned long long b) { retur
 // ---------------------------------------------------------------------------
 // Edge weights: per-point value loaded once per row/column, then a symmetric
 // combination with the pair distance (reference src/distance_matrix.cuh:202-219,
-// src/hdbscan.cuh:342-351).
+// src/hdbscan.cuh:342-351). distBound(w) is an upper bound on every distance
+// whose weight is <= w (for any per-point values).
 
 struct PlainWeight {
   using Value = float;
   DEV Value value(int) const { return 0.0f; }
   DEV float operator()(float dist, Value, Value) const { return dist; }
+  DEV float distBound(float w) const { return w; }
   static constexpr bool kHasValue = false;
 };
 
@@ -163,6 +183,10 @@ template <bool kScaled> struct MReachWeightT {
   DEV Value value(int p) const { return core[p]; }
   DEV float operator()(float dist, Value ci, Value cj) const {
     return fmaxf(fmaxf(ci, cj), kScaled ? dist / alpha : dist);
+  }
+  // fl(dist / alpha) <= w implies dist <= w alpha (1 + 2^-24).
+  DEV float distBound(float w) const {
+    return kScaled ? w * alpha * (1.0f + 0x1p-21f) + FLT_MIN : w;
   }
   static constexpr bool kHasValue = true;
 };
@@ -177,6 +201,7 @@ struct WardWeight {
   DEV float operator()(float sq, Value si, Value sj) const {
     return (2.0f * (si * sj)) / (si + sj) * sq;
   }
+  DEV float distBound(float) const { return INFINITY; }
   static constexpr bool kHasValue = true;
 };
 

@@ -120,8 +120,12 @@ __global__ void __launch_bounds__(256)
         continue;
     }
     const int col0 = colTile * T::DIM;
+    // With an accumulator bound the tile holds raw accumulators, and only the
+    // pairs within the bound of the row's current k-th distance are finished
+    // (most pairs of a tile are not, which skips their sqrt).
+    constexpr bool kFilter = Metric::kAccFilter;
     float v[TR][TR];
-    computeTile<T>(sm, A, nA, X, n, d, metric, row0, col0, v);
+    computeTile<T, Metric, !kFilter>(sm, A, nA, X, n, d, metric, row0, col0, v);
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
       const int rit = T::rowInTile(r);
@@ -131,13 +135,28 @@ __global__ void __launch_bounds__(256)
       if (row < nA) {
         const float kth = list[k - 1];
         const int self = rowsX ? rowPos[row] : row;
+        if constexpr (kFilter) {
+          const float limit = metric.accBound(kth);
 #pragma unroll
-        for (int c = 0; c < TR; ++c) {
-          const int col = col0 + T::colInTile(c);
-          // The point itself is distance 0 (drops float noise such as cosine's 1e-7).
-          v[r][c] = col == self ? 0.0f : fmaxf(v[r][c], 0.0f);
-          if (col < n && v[r][c] < kth)
-            pending |= 1u << c;
+          for (int c = 0; c < TR; ++c) {
+            const int col = col0 + T::colInTile(c);
+            const float acc = v[r][c];
+            if (col < n && !(acc > limit)) {
+              const float value = col == self ? 0.0f : fmaxf(metric.finish(acc), 0.0f);
+              v[r][c] = value;
+              if (value < kth)
+                pending |= 1u << c;
+            }
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < TR; ++c) {
+            const int col = col0 + T::colInTile(c);
+            // The point itself is distance 0 (drops float noise such as cosine's 1e-7).
+            v[r][c] = col == self ? 0.0f : fmaxf(v[r][c], 0.0f);
+            if (col < n && v[r][c] < kth)
+              pending |= 1u << c;
+          }
         }
       }
       // One insertion per row per step until no lane of the wave has work;

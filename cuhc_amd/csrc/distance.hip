@@ -287,6 +287,10 @@ DEV int walkTile(int rowTile, int step, int numTiles) {
 // the weight bits; null disables it). The row that holds the component's
 // smallest edge always finds it exactly; other rows may stop early, and their
 // results are then only upper bounds, which the contraction never prefers.
+// The same need filters single pairs: wave 0 publishes each row's need in
+// LDS, and for metrics with an accumulator bound a pair is finished (sqrt,
+// weight, key) only when its raw accumulator is within the row's need and
+// it crosses components; the pairs filtered out weigh more than the need.
 template <class T, class Metric, class Weight>
 __global__ void __launch_bounds__(256)
     nearestOtherWalkKernel(const float *__restrict__ X, int n, int d, Metric metric,
@@ -300,14 +304,18 @@ __global__ void __launch_bounds__(256)
   constexpr int kBatch = 64;
   __shared__ __align__(16) TileSmem<T> sm;
   __shared__ unsigned long long rowBest[T::DIM];
+  __shared__ float rowNeed[T::DIM]; // -1: no row
   __shared__ float lowerBound[kBatch];
   __shared__ float bound;
   extern __shared__ float rowBox[]; // low[d], high[d]
+  constexpr bool kFilter = Metric::kAccFilter;
 
   const int rowTile = blockIdx.x * rowTileStride, numTiles = boxes.numTiles;
   const int row0 = rowTile * T::DIM;
-  for (int i = threadIdx.x; i < T::DIM; i += T::THREADS)
+  for (int i = threadIdx.x; i < T::DIM; i += T::THREADS) {
     rowBest[i] = kEmptyKey;
+    rowNeed[i] = row0 + i < n ? INFINITY : -1.0f;
+  }
   for (int k = threadIdx.x; k < d; k += T::THREADS) {
     rowBox[k] = boxes.low[(size_t)rowTile * d + k];
     rowBox[d + k] = boxes.high[(size_t)rowTile * d + k];
@@ -338,7 +346,8 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
 
   // The bound: the largest need over the open rows (inf while a row of a
-  // component without a known edge has no candidate yet). Wave 0 only.
+  // component without a known edge has no candidate yet). Wave 0 only; it
+  // also publishes each row's need.
   auto newBound = [&]() {
     float need = 0.0f;
     if (open0) {
@@ -349,6 +358,7 @@ __global__ void __launch_bounds__(256)
         cap0 = w;
       }
       need = fminf(w, cap0);
+      rowNeed[threadIdx.x] = need;
     }
     if (open1) {
       const unsigned long long k = rowBest[threadIdx.x + 64];
@@ -357,7 +367,9 @@ __global__ void __launch_bounds__(256)
         atomicMin(&compBest[comp1], bitsOf(w));
         cap1 = w;
       }
-      need = fmaxf(need, fminf(w, cap1));
+      const float need1 = fminf(w, cap1);
+      rowNeed[threadIdx.x + 64] = need1;
+      need = fmaxf(need, need1);
     }
 #pragma unroll
     for (int m = 32; m > 0; m >>= 1)
@@ -401,8 +413,8 @@ __global__ void __launch_bounds__(256)
       if (base + j >= computeSteps)
         continue; // estimate: counted against the bound reached so far, not computed
       const int col0 = colTile * T::DIM;
-      float w[TR][TR];
-      computeTile<T>(sm, X, n, X, n, d, metric, row0, col0, w);
+      float w[TR][TR]; // raw accumulators when filtering
+      computeTile<T, Metric, !kFilter>(sm, X, n, X, n, d, metric, row0, col0, w);
       int colComp[TR];
       typename Weight::Value colVal[TR];
       unsigned int colIdx[TR];
@@ -416,18 +428,36 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int r = 0; r < TR; ++r) {
         unsigned long long best = kEmptyKey;
+        if constexpr (kFilter) {
+          // rowNeed was published before this tile's barriers.
+          const float need = rowNeed[T::rowInTile(r)];
+          const float limit = need < 0.0f ? -INFINITY : metric.accBound(weight.distBound(need));
 #pragma unroll
-        for (int c = 0; c < TR; ++c) {
-          const float ww = weight(w[r][c], rowVal[r], colVal[c]);
-          const bool counts = rowComp[r] >= 0 && colComp[c] >= 0 && colComp[c] != rowComp[r] &&
-                              ww != INFINITY;
-          if (counts)
-            best = minKey(best, packKey(ww, colIdx[c]));
+          for (int c = 0; c < TR; ++c) {
+            const float acc = w[r][c];
+            if (!(acc > limit) && colComp[c] != rowComp[r]) {
+              const float ww = weight(metric.finish(acc), rowVal[r], colVal[c]);
+              if (rowComp[r] >= 0 && colComp[c] >= 0 && ww != INFINITY)
+                best = minKey(best, packKey(ww, colIdx[c]));
+            }
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < TR; ++c) {
+            const float ww = weight(w[r][c], rowVal[r], colVal[c]);
+            const bool counts = rowComp[r] >= 0 && colComp[c] >= 0 &&
+                                colComp[c] != rowComp[r] && ww != INFINITY;
+            if (counts)
+              best = minKey(best, packKey(ww, colIdx[c]));
+          }
         }
-        best = rowGroupMin(best);
-        if (T::tc() == 0 && best != kEmptyKey) {
-          const int i = T::rowInTile(r);
-          rowBest[i] = minKey(rowBest[i], best);
+        // Wave-uniform: most rows of most tiles have no candidate.
+        if (__ballot(best != kEmptyKey) != 0) {
+          best = rowGroupMin(best);
+          if (T::tc() == 0 && best != kEmptyKey) {
+            const int i = T::rowInTile(r);
+            rowBest[i] = minKey(rowBest[i], best);
+          }
         }
       }
       __syncthreads();

@@ -179,18 +179,23 @@ __global__ void __launch_bounds__(256)
       continue;
     ++computed;
     const int col0 = colTile * T::DIM;
+    // Raw accumulators: only the pairs within the bound of the row's radius
+    // are finished (sqrt) and tested exactly.
     float v[TR][TR];
-    computeTile<T>(sm, X, n, X, n, d, metric, row0, col0, v);
+    computeTile<T, Metric, false>(sm, X, n, X, n, d, metric, row0, col0, v);
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
       const int rit = T::rowInTile(r), row = row0 + rit;
       if (row >= n)
         continue;
+      const float limit = metric.accBound(rRow[r]);
 #pragma unroll
       for (int c = 0; c < TR; ++c) {
         const int col = col0 + T::colInTile(c);
-        const float val = col == row ? 0.0f : fmaxf(v[r][c], 0.0f);
-        if (col < n && val <= rRow[r]) {
+        if (col >= n || v[r][c] > limit)
+          continue;
+        const float val = col == row ? 0.0f : fmaxf(metric.finish(v[r][c]), 0.0f);
+        if (val <= rRow[r]) {
           const int pos = atomicAdd(&cnt[rit], 1);
           if (pos < kCandPerRow)
             cand[(size_t)row * kCandPerRow + pos] = val;

@@ -51,10 +51,11 @@ DEV void stageSlice(float (*s)[T::LD], const float *__restrict__ pts, int n, int
   }
 }
 
-// out[r][c] = metric(A[rowA0 + rowInTile(r)], B[colB0 + colInTile(c)]).
+// out[r][c] = metric(A[rowA0 + rowInTile(r)], B[colB0 + colInTile(c)]), or
+// the raw accumulator when kFinish is false (scalar accumulators only).
 // Entries outside the matrices are garbage. Contains __syncthreads(); every
 // thread of the workgroup must call it.
-template <class T, class Metric>
+template <class T, class Metric, bool kFinish = true>
 DEV void computeTile(TileSmem<T> &sm, const float *__restrict__ A, int nA,
                      const float *__restrict__ B, int nB, int d, const Metric &metric, int rowA0,
                      int colB0, float (&out)[T::TR][T::TR]) {
@@ -99,8 +100,12 @@ DEV void computeTile(TileSmem<T> &sm, const float *__restrict__ A, int nA,
 #pragma unroll
   for (int r = 0; r < TR; ++r)
 #pragma unroll
-    for (int c = 0; c < TR; ++c)
-      out[r][c] = metric.finish(acc[r][c]);
+    for (int c = 0; c < TR; ++c) {
+      if constexpr (kFinish)
+        out[r][c] = metric.finish(acc[r][c]);
+      else
+        out[r][c] = acc[r][c];
+    }
 }
 
 // Tile geometry per metric: wide (cosine) accumulators get 4x4 outputs per
