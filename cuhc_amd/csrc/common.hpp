@@ -172,6 +172,7 @@ struct PlainWeight {
   DEV float operator()(float dist, Value, Value) const { return dist; }
   DEV float distBound(float w) const { return w; }
   static constexpr bool kHasValue = false;
+  static constexpr bool kDistBound = true;
 };
 
 // kScaled: alpha != 1 (the division is a separate instantiation: a select
@@ -189,6 +190,7 @@ template <bool kScaled> struct MReachWeightT {
     return kScaled ? w * alpha * (1.0f + 0x1p-21f) + FLT_MIN : w;
   }
   static constexpr bool kHasValue = true;
+  static constexpr bool kDistBound = true;
 };
 
 // Ward's weight for the nearest-neighbour choice between cluster centroids:
@@ -203,6 +205,7 @@ struct WardWeight {
   }
   DEV float distBound(float) const { return INFINITY; }
   static constexpr bool kHasValue = true;
+  static constexpr bool kDistBound = false; // the weight scales the distance by cluster sizes
 };
 
 // ---------------------------------------------------------------------------

@@ -111,6 +111,17 @@ DEV unsigned long long rowGroupMin(unsigned long long k) {
   return k;
 }
 
+// Accumulator limit of a point for the pair filter: the bound of the weight
+// of its current nearest (+inf while it has none). The weight is the key's
+// high word, read alone; nearest only decreases, so a stale read is a looser
+// limit.
+template <class Metric, class Weight>
+DEV float pointLimit(const Metric &metric, const Weight &weight,
+                     const unsigned long long *nearest, int p) {
+  const unsigned int bits = reinterpret_cast<const unsigned int *>(nearest)[2 * (size_t)p + 1];
+  return bits == 0xffffffffu ? INFINITY : metric.accBound(weight.distBound(floatOf(bits)));
+}
+
 template <class T, class Metric, class Weight>
 __global__ void __launch_bounds__(256)
     nearestOtherSymmetricKernel(const float *__restrict__ X, int n, int d, Metric metric,
@@ -118,6 +129,12 @@ __global__ void __launch_bounds__(256)
                                 const int *__restrict__ orig,
                                 unsigned long long *__restrict__ nearest, long long numPairs) {
   constexpr int TR = T::TR;
+  // With an accumulator bound, a pair is finished (sqrt, weight, keys) only
+  // when its raw accumulator is within the limit of its row point or of its
+  // column point (from their current nearest) and it crosses components.
+  // Every point's nearest pair is within its own limit, so every point's
+  // result stays exact.
+  constexpr bool kFilter = Metric::kAccFilter && Weight::kDistBound;
   __shared__ __align__(16) TileSmem<T> sm;
   __shared__ unsigned long long colBest[T::DIM];
 
@@ -128,8 +145,8 @@ __global__ void __launch_bounds__(256)
     for (int i = threadIdx.x; i < T::DIM; i += T::THREADS)
       colBest[i] = kEmptyKey; // visible after computeTile's barriers
 
-    float w[TR][TR];
-    computeTile<T>(sm, X, n, X, n, d, metric, row0, col0, w);
+    float w[TR][TR]; // raw accumulators when filtering
+    computeTile<T, Metric, !kFilter>(sm, X, n, X, n, d, metric, row0, col0, w);
 
     int colComp[TR];
     unsigned int colId[TR];
@@ -146,23 +163,48 @@ __global__ void __launch_bounds__(256)
     // symmetric pass; NaN counts and packs as 0).
     unsigned long long rowKey[TR];
     unsigned long long countMask = 0;
+    float colLimit[TR];
+    if constexpr (kFilter) {
+#pragma unroll
+      for (int c = 0; c < TR; ++c) {
+        const int col = col0 + T::colInTile(c);
+        colLimit[c] = col < n ? pointLimit(metric, weight, nearest, col) : -INFINITY;
+      }
+    }
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
       const int row = row0 + T::rowInTile(r);
       const int rc = row < n ? comp[row] : -2;
       const typename Weight::Value rv = weight.value(row < n ? row : 0);
       unsigned long long best = kEmptyKey;
+      if constexpr (kFilter) {
+        const float rowLimit = row < n ? pointLimit(metric, weight, nearest, row) : -INFINITY;
 #pragma unroll
-      for (int c = 0; c < TR; ++c) {
-        const float ww = weight(w[r][c], rv, colVal[c]);
-        const bool counts = colComp[c] >= 0 && rc >= 0 && colComp[c] != rc && ww != INFINITY;
-        w[r][c] = ww;
-        if (counts) {
-          countMask |= 1ull << (r * TR + c);
-          best = minKey(best, packKey(ww, colId[c]));
+        for (int c = 0; c < TR; ++c) {
+          const float acc = w[r][c];
+          if (!(acc > fmaxf(rowLimit, colLimit[c])) && colComp[c] != rc) {
+            const float ww = weight(metric.finish(acc), rv, colVal[c]);
+            w[r][c] = ww;
+            if (colComp[c] >= 0 && rc >= 0 && ww != INFINITY) {
+              countMask |= 1ull << (r * TR + c);
+              best = minKey(best, packKey(ww, colId[c]));
+            }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < TR; ++c) {
+          const float ww = weight(w[r][c], rv, colVal[c]);
+          const bool counts = colComp[c] >= 0 && rc >= 0 && colComp[c] != rc && ww != INFINITY;
+          w[r][c] = ww;
+          if (counts) {
+            countMask |= 1ull << (r * TR + c);
+            best = minKey(best, packKey(ww, colId[c]));
+          }
         }
       }
-      rowKey[r] = rowGroupMin(best);
+      // Wave-uniform: skip the reduction when no lane has a candidate.
+      rowKey[r] = __ballot(best != kEmptyKey) != 0 ? rowGroupMin(best) : kEmptyKey;
     }
     if (T::tc() == 0) {
 #pragma unroll
@@ -174,24 +216,26 @@ __global__ void __launch_bounds__(256)
     }
     if (I != J) {
       // Columns: this thread's rows, then lanes tc+16, +32, +48 of the wave,
-      // then the 4 waves through LDS.
-      unsigned int rowId[TR];
+      // then the 4 waves through LDS (skipped by a wave with no counted pair).
+      if (__ballot(countMask != 0) != 0) {
+        unsigned int rowId[TR];
 #pragma unroll
-      for (int r = 0; r < TR; ++r) {
-        const int row = row0 + T::rowInTile(r);
-        rowId[r] = (unsigned)(orig && row < n ? orig[row] : row);
-      }
+        for (int r = 0; r < TR; ++r) {
+          const int row = row0 + T::rowInTile(r);
+          rowId[r] = (unsigned)(orig && row < n ? orig[row] : row);
+        }
 #pragma unroll
-      for (int c = 0; c < TR; ++c) {
-        unsigned long long best = kEmptyKey;
+        for (int c = 0; c < TR; ++c) {
+          unsigned long long best = kEmptyKey;
 #pragma unroll
-        for (int r = 0; r < TR; ++r)
-          if (countMask & (1ull << (r * TR + c)))
-            best = minKey(best, packKey(w[r][c], rowId[r]));
-        best = minKey(best, shflXorKey(best, 16));
-        best = minKey(best, shflXorKey(best, 32));
-        if ((threadIdx.x & 63) < 16 && best != kEmptyKey)
-          atomicMin(&colBest[T::colInTile(c)], best);
+          for (int r = 0; r < TR; ++r)
+            if (countMask & (1ull << (r * TR + c)))
+              best = minKey(best, packKey(w[r][c], rowId[r]));
+          best = minKey(best, shflXorKey(best, 16));
+          best = minKey(best, shflXorKey(best, 32));
+          if ((threadIdx.x & 63) < 16 && best != kEmptyKey)
+            atomicMin(&colBest[T::colInTile(c)], best);
+        }
       }
       __syncthreads();
       for (int i = threadIdx.x; i < T::DIM; i += T::THREADS) {
@@ -308,7 +352,7 @@ __global__ void __launch_bounds__(256)
   __shared__ float lowerBound[kBatch];
   __shared__ float bound;
   extern __shared__ float rowBox[]; // low[d], high[d]
-  constexpr bool kFilter = Metric::kAccFilter;
+  constexpr bool kFilter = Metric::kAccFilter && Weight::kDistBound;
 
   const int rowTile = blockIdx.x * rowTileStride, numTiles = boxes.numTiles;
   const int row0 = rowTile * T::DIM;
