@@ -13,6 +13,7 @@
 #include <atomic>
 #include <thread>
 #include <numeric>
+#include <type_traits>
 
 namespace cuhc {
 
@@ -111,19 +112,40 @@ DEV unsigned long long rowGroupMin(unsigned long long k) {
   return k;
 }
 
-// Accumulator limit of a point for the pair filter: the bound of the weight
-// of its current nearest (+inf while it has none). The weight is the key's
-// high word, read alone; nearest only decreases, so a stale read is a looser
-// limit.
-template <class Metric, class Weight>
-DEV float pointLimit(const Metric &metric, const Weight &weight,
-                     const unsigned long long *nearest, int p) {
-  const unsigned int bits = reinterpret_cast<const unsigned int *>(nearest)[2 * (size_t)p + 1];
-  return bits == 0xffffffffu ? INFINITY : metric.accBound(weight.distBound(floatOf(bits)));
-}
+// Pair filter of the symmetric pass. A point's limit comes from the weight of
+// its current nearest (+inf while it has none; the key's high word, read
+// alone: nearest only decreases, so a stale read is a looser limit). A pair
+// passes when its filter value is within the larger of its two points'
+// limits; every pair whose weight is within a point's nearest weight passes.
+//   metrics with an accumulator bound (plain and mutual-reachability
+//   weights): value = the raw accumulator, limit = accBound(distBound(w));
+//   Ward: 2 si sj / (si + sj) >= min(si, sj), so the weight is at least
+//   sq min(si, sj) up to the rounding of its four operations (2^-22):
+//   value = sq min(si, sj), limit = w (1 + 2^-20) + FLT_MIN.
+template <class Metric, class Weight> struct PairFilter {
+  static constexpr bool kWard = std::is_same<Weight, WardWeight>::value;
+  static constexpr bool kOn = (Metric::kAccFilter && Weight::kDistBound) || kWard;
+  DEV static float limit(const Metric &metric, const Weight &weight,
+                         const unsigned long long *nearest, int p) {
+    const unsigned int bits = reinterpret_cast<const unsigned int *>(nearest)[2 * (size_t)p + 1];
+    if (bits == 0xffffffffu)
+      return INFINITY;
+    const float w = floatOf(bits);
+    if constexpr (kWard)
+      return w * (1.0f + 0x1p-20f) + FLT_MIN;
+    else
+      return metric.accBound(weight.distBound(w));
+  }
+  DEV static float value(float acc, typename Weight::Value vi, typename Weight::Value vj) {
+    if constexpr (kWard)
+      return acc * fminf(vi, vj);
+    else
+      return acc;
+  }
+};
 
 template <class T, class Metric, class Weight>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 3) // three waves per SIMD (at most 168 VGPRs)
     nearestOtherSymmetricKernel(const float *__restrict__ X, int n, int d, Metric metric,
                                 Weight weight, const int *__restrict__ comp,
                                 const int *__restrict__ orig,
@@ -134,7 +156,8 @@ __global__ void __launch_bounds__(256)
   // column point (from their current nearest) and it crosses components.
   // Every point's nearest pair is within its own limit, so every point's
   // result stays exact.
-  constexpr bool kFilter = Metric::kAccFilter && Weight::kDistBound;
+  using Filter = PairFilter<Metric, Weight>;
+  constexpr bool kFilter = Filter::kOn;
   __shared__ __align__(16) TileSmem<T> sm;
   __shared__ unsigned long long colBest[T::DIM];
 
@@ -168,7 +191,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int c = 0; c < TR; ++c) {
         const int col = col0 + T::colInTile(c);
-        colLimit[c] = col < n ? pointLimit(metric, weight, nearest, col) : -INFINITY;
+        colLimit[c] = col < n ? Filter::limit(metric, weight, nearest, col) : -INFINITY;
       }
     }
 #pragma unroll
@@ -178,11 +201,12 @@ __global__ void __launch_bounds__(256)
       const typename Weight::Value rv = weight.value(row < n ? row : 0);
       unsigned long long best = kEmptyKey;
       if constexpr (kFilter) {
-        const float rowLimit = row < n ? pointLimit(metric, weight, nearest, row) : -INFINITY;
+        const float rowLimit = row < n ? Filter::limit(metric, weight, nearest, row) : -INFINITY;
 #pragma unroll
         for (int c = 0; c < TR; ++c) {
           const float acc = w[r][c];
-          if (!(acc > fmaxf(rowLimit, colLimit[c])) && colComp[c] != rc) {
+          if (!(Filter::value(acc, rv, colVal[c]) > fmaxf(rowLimit, colLimit[c])) &&
+              colComp[c] != rc) {
             const float ww = weight(metric.finish(acc), rv, colVal[c]);
             w[r][c] = ww;
             if (colComp[c] >= 0 && rc >= 0 && ww != INFINITY) {
