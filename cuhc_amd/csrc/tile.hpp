@@ -40,14 +40,38 @@ template <class T> struct TileSmem {
 };
 
 // Copies features [f0, f0 + nf8) of points [p0, p0 + DIM) into s[k][p]
-// (zero outside the matrix or past d).
+// (zero outside the matrix or past d). A thread issues its loads in batches
+// of up to four before storing them to LDS, so a fill costs one to four
+// memory round trips instead of one per element. idx / nf8 is
+// (idx * magic) >> 20 with magic = ceil(2^20 / nf8): exact for idx < 2^12
+// and nf8 <= 32 (the rounding adds less than 2^-8, below the 1 / nf8 gap to
+// the next integer).
 template <class T>
 DEV void stageSlice(float (*s)[T::LD], const float *__restrict__ pts, int n, int d, int p0, int f0,
                     int nf8) {
-  for (int idx = threadIdx.x; idx < T::DIM * nf8; idx += T::THREADS) {
-    const int p = idx / nf8, k = idx - (idx / nf8) * nf8;
-    const int point = p0 + p, f = f0 + k;
-    s[k][p] = (point < n && f < d) ? pts[(size_t)point * d + f] : 0.0f;
+  constexpr int kPer = T::DIM * T::FILL / T::THREADS;
+  constexpr int kBatch = kPer < 4 ? kPer : 4;
+  static_assert(T::DIM * T::FILL <= 4096 && T::FILL <= 32 && kPer % kBatch == 0,
+                "division by multiplication, batches");
+  const int total = T::DIM * nf8;
+  const unsigned magic = ((1u << 20) + (unsigned)nf8 - 1u) / (unsigned)nf8;
+#pragma unroll
+  for (int b = 0; b < kPer; b += kBatch) {
+    float v[kBatch];
+#pragma unroll
+    for (int i = 0; i < kBatch; ++i) {
+      const int idx = (int)threadIdx.x + (b + i) * T::THREADS;
+      const int p = (int)(((unsigned)idx * magic) >> 20), k = idx - p * nf8;
+      const int point = p0 + p, f = f0 + k;
+      v[i] = (idx < total && point < n && f < d) ? pts[(size_t)point * d + f] : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < kBatch; ++i) {
+      const int idx = (int)threadIdx.x + (b + i) * T::THREADS;
+      const int p = (int)(((unsigned)idx * magic) >> 20), k = idx - p * nf8;
+      if (idx < total)
+        s[k][p] = v[i];
+    }
   }
 }
 
