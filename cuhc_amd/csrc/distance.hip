@@ -375,6 +375,7 @@ __global__ void __launch_bounds__(256)
   __shared__ float rowNeed[T::DIM]; // -1: no row
   __shared__ float lowerBound[kBatch];
   __shared__ float bound;
+  __shared__ int tileImproved; // a row's best improved in the current tile
   extern __shared__ float rowBox[]; // low[d], high[d]
   constexpr bool kFilter = Metric::kAccFilter && Weight::kDistBound;
 
@@ -395,8 +396,10 @@ __global__ void __launch_bounds__(256)
   const int comp0 = open0 ? comp[ra] : 0, comp1 = open1 ? comp[rb] : 0;
   float cap0 = INFINITY, cap1 = INFINITY;
   auto capOf = [&](unsigned int bits) { return bits == 0xffffffffu ? INFINITY : floatOf(bits); };
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
     bound = INFINITY;
+    tileImproved = 0;
+  }
   const bool rowTilePure = boxes.minComp[rowTile] == boxes.maxComp[rowTile];
   const int rowCompTile = boxes.minComp[rowTile];
   const float rowMinValue = boxes.minValue[rowTile];
@@ -524,17 +527,29 @@ __global__ void __launch_bounds__(256)
           best = rowGroupMin(best);
           if (T::tc() == 0 && best != kEmptyKey) {
             const int i = T::rowInTile(r);
-            rowBest[i] = minKey(rowBest[i], best);
+            if (best < rowBest[i]) {
+              rowBest[i] = best;
+              tileImproved = 1;
+            }
           }
         }
       }
       __syncthreads();
-      if (threadIdx.x < 64) {
-        const float b = newBound();
+      // The needs and the bound change only with a row's best: a tile that
+      // improved none skips the update and its barrier. Every wave reads the
+      // flag after the barrier above; it is cleared only after the barrier
+      // below (all reads done) and set again only after the next tile's
+      // first barrier, so every wave takes the same branch.
+      if (tileImproved) {
+        if (threadIdx.x < 64) {
+          const float b = newBound();
+          if (threadIdx.x == 0)
+            bound = b;
+        }
+        __syncthreads();
         if (threadIdx.x == 0)
-          bound = b;
+          tileImproved = 0;
       }
-      __syncthreads();
     }
     __syncthreads(); // lowerBound is rewritten by the next batch
   }
