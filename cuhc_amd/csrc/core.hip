@@ -68,8 +68,11 @@ __global__ void __launch_bounds__(256)
                           const float *__restrict__ rowsX, int numRows,
                           const int *__restrict__ rowPos, const int *__restrict__ tileList) {
   constexpr int TR = T::TR;
+  constexpr int kBatch = 64; // walk steps whose box bounds wave 0 computes at once
   __shared__ __align__(16) TileSmem<T> sm;
-  __shared__ int skip;
+  __shared__ float lowerBound[kBatch];
+  __shared__ float largestKth; // the largest k-th distance over the tile's rows
+  __shared__ int inserted;     // a row's list changed in the current tile
   extern __shared__ float lists[]; // [DIM][k], then the row tile's box (2d) if pruning
 
   const int rowTile = tileList ? tileList[blockIdx.x] : (int)blockIdx.x;
@@ -85,6 +88,10 @@ __global__ void __launch_bounds__(256)
       rowBox[f] = boxes.low[(size_t)rowTile * d + f];
       rowBox[d + f] = boxes.high[(size_t)rowTile * d + f];
     }
+  if (threadIdx.x == 0) {
+    largestKth = INFINITY; // every list starts empty
+    inserted = 0;
+  }
   __syncthreads();
   const float slack = 1.0f - coreBoxSlack(d);
   const int numTiles = (n + T::DIM - 1) / T::DIM;
@@ -93,94 +100,123 @@ __global__ void __launch_bounds__(256)
   // Listed rows have no home tile: the walk starts anywhere and covers all.
   const int homeTile = rowsX ? (int)(blockIdx.x % numTiles) : rowTile;
 
+  // A step is skipped when the box bound between the row tile and its column
+  // tile exceeds the largest k-th distance of the tile's rows: wave 0 computes
+  // the bounds of 64 steps at once into LDS, and the largest k-th distance
+  // is refreshed only after a tile that changed a list, so skipped steps and
+  // unchanged tiles cost no barrier.
   const int steps = min(2 * numTiles, stepLimit);
-  for (int step = 0; step < steps; ++step) {
-    const int colTile = walkTileCore(homeTile, step, numTiles);
-    if (colTile < 0)
-      continue;
+  for (int base = 0; base < steps; base += kBatch) {
     if (prune) {
-      __syncthreads(); // previous insertions done
-      if (threadIdx.x < 64) {
-        float largest = 0.0f;
-        for (int i = threadIdx.x; i < T::DIM; i += 64)
-          if (row0 + i < nA)
-            largest = fmaxf(largest, lists[i * k + k - 1]);
-#pragma unroll
-        for (int m = 32; m > 0; m >>= 1)
-          largest = fmaxf(largest, __shfl_xor(largest, m, 64));
-        if (threadIdx.x == 0)
-          skip = largest != INFINITY &&
-                 coreBoxBound(metric, rowBox, rowBox + d, boxes.low + (size_t)colTile * d,
-                              boxes.high + (size_t)colTile * d, d) * slack > largest;
+      if (threadIdx.x < kBatch) {
+        const int step = base + (int)threadIdx.x;
+        const int colTile = step < steps ? walkTileCore(homeTile, step, numTiles) : -1;
+        lowerBound[threadIdx.x] =
+            colTile >= 0 ? coreBoxBound(metric, rowBox, rowBox + d, boxes.low + (size_t)colTile * d,
+                                        boxes.high + (size_t)colTile * d, d) * slack
+                         : 0.0f;
       }
       __syncthreads();
-      const bool doSkip = skip;
-      __syncthreads();
-      if (doSkip)
-        continue;
     }
-    const int col0 = colTile * T::DIM;
-    // With an accumulator bound the tile holds raw accumulators, and only the
-    // pairs within the bound of the row's current k-th distance are finished
-    // (most pairs of a tile are not, which skips their sqrt).
-    constexpr bool kFilter = Metric::kAccFilter;
-    float v[TR][TR];
-    computeTile<T, Metric, !kFilter>(sm, A, nA, X, n, d, metric, row0, col0, v);
+    const int batchEnd = min(steps, base + kBatch);
+    for (int step = base; step < batchEnd; ++step) {
+      const int colTile = walkTileCore(homeTile, step, numTiles);
+      if (colTile < 0)
+        continue;
+      if (prune) {
+        const float largest = largestKth; // uniform: LDS, written only between barriers
+        if (largest != INFINITY && lowerBound[step - base] > largest)
+          continue;
+      }
+      const int col0 = colTile * T::DIM;
+      // With an accumulator bound the tile holds raw accumulators, and only the
+      // pairs within the bound of the row's current k-th distance are finished
+      // (most pairs of a tile are not, which skips their sqrt).
+      constexpr bool kFilter = Metric::kAccFilter;
+      float v[TR][TR];
+      computeTile<T, Metric, !kFilter>(sm, A, nA, X, n, d, metric, row0, col0, v);
 #pragma unroll
-    for (int r = 0; r < TR; ++r) {
-      const int rit = T::rowInTile(r);
-      const int row = row0 + rit;
-      float *list = lists + rit * k;
-      unsigned int pending = 0;
-      if (row < nA) {
-        const float kth = list[k - 1];
-        const int self = rowsX ? rowPos[row] : row;
-        if constexpr (kFilter) {
-          const float limit = metric.accBound(kth);
+      for (int r = 0; r < TR; ++r) {
+        const int rit = T::rowInTile(r);
+        const int row = row0 + rit;
+        float *list = lists + rit * k;
+        unsigned int pending = 0;
+        if (row < nA) {
+          const float kth = list[k - 1];
+          const int self = rowsX ? rowPos[row] : row;
+          if constexpr (kFilter) {
+            const float limit = metric.accBound(kth);
 #pragma unroll
-          for (int c = 0; c < TR; ++c) {
-            const int col = col0 + T::colInTile(c);
-            const float acc = v[r][c];
-            if (col < n && !(acc > limit)) {
-              const float value = col == self ? 0.0f : fmaxf(metric.finish(acc), 0.0f);
-              v[r][c] = value;
-              if (value < kth)
+            for (int c = 0; c < TR; ++c) {
+              const int col = col0 + T::colInTile(c);
+              const float acc = v[r][c];
+              if (col < n && !(acc > limit)) {
+                const float value = col == self ? 0.0f : fmaxf(metric.finish(acc), 0.0f);
+                v[r][c] = value;
+                if (value < kth)
+                  pending |= 1u << c;
+              }
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < TR; ++c) {
+              const int col = col0 + T::colInTile(c);
+              // The point itself is distance 0 (drops float noise such as cosine's 1e-7).
+              v[r][c] = col == self ? 0.0f : fmaxf(v[r][c], 0.0f);
+              if (col < n && v[r][c] < kth)
                 pending |= 1u << c;
             }
           }
-        } else {
+        }
+        // One insertion per row per step until no lane of the wave has work;
+        // all lanes stay converged for the ballot.
+        while (true) {
+          const unsigned long long work = __ballot(pending != 0);
+          if (work == 0)
+            break;
+          const unsigned long long mine = work & groupMask;
+          if (mine != 0 && lane == __ffsll((long long)mine) - 1) {
+            const int c = __ffs(pending) - 1;
+            pending &= pending - 1;
+            float value = 0.0f;
 #pragma unroll
-          for (int c = 0; c < TR; ++c) {
-            const int col = col0 + T::colInTile(c);
-            // The point itself is distance 0 (drops float noise such as cosine's 1e-7).
-            v[r][c] = col == self ? 0.0f : fmaxf(v[r][c], 0.0f);
-            if (col < n && v[r][c] < kth)
-              pending |= 1u << c;
+            for (int cc = 0; cc < TR; ++cc)
+              if (cc == c)
+                value = v[r][cc];
+            insertSorted(list, k, value);
+            if (prune)
+              inserted = 1;
           }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
       }
-      // One insertion per row per step until no lane of the wave has work;
-      // all lanes stay converged for the ballot.
-      while (true) {
-        const unsigned long long work = __ballot(pending != 0);
-        if (work == 0)
-          break;
-        const unsigned long long mine = work & groupMask;
-        if (mine != 0 && lane == __ffsll((long long)mine) - 1) {
-          const int c = __ffs(pending) - 1;
-          pending &= pending - 1;
-          float value = 0.0f;
+      if (prune) {
+        __syncthreads(); // insertions done
+        // Every wave reads the flag after the barrier above; it is cleared
+        // after the barrier below and set again only after the next tile's
+        // first barrier, so every wave takes the same branch.
+        if (inserted) {
+          if (threadIdx.x < 64) {
+            float largest = 0.0f;
+            for (int i = threadIdx.x; i < T::DIM; i += 64)
+              if (row0 + i < nA)
+                largest = fmaxf(largest, lists[i * k + k - 1]);
 #pragma unroll
-          for (int cc = 0; cc < TR; ++cc)
-            if (cc == c)
-              value = v[r][cc];
-          insertSorted(list, k, value);
+            for (int m = 32; m > 0; m >>= 1)
+              largest = fmaxf(largest, __shfl_xor(largest, m, 64));
+            if (threadIdx.x == 0)
+              largestKth = largest;
+          }
+          __syncthreads();
+          if (threadIdx.x == 0)
+            inserted = 0;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
     }
+    if (prune)
+      __syncthreads(); // lowerBound is rewritten by the next batch
   }
   __syncthreads();
   for (int i = threadIdx.x; i < T::DIM; i += T::THREADS)
