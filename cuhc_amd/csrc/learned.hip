@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(256)
   __shared__ __align__(16) TileSmem<T> sm;
   __shared__ int cnt[T::DIM];
   __shared__ float rmaxWave[4];
-  __shared__ int skip;
+  __shared__ bool skipStep[64];
   extern __shared__ float rowBox[]; // low[d], high[d]
   const int numTiles = (n + T::DIM - 1) / T::DIM;
   const int rowTile = blockIdx.x, row0 = rowTile * T::DIM;
@@ -165,43 +165,50 @@ __global__ void __launch_bounds__(256)
   const float slack = 1.0f - learnedSlack(d);
   unsigned long long computed = 0;
 
-  for (int step = 0; step < 2 * numTiles; ++step) {
-    const int colTile = walkStep(rowTile, step, numTiles);
-    if (colTile < 0)
-      continue;
-    if (threadIdx.x == 0)
-      skip = boxBound(metric, rowBox, rowBox + d, low + (size_t)colTile * d,
-                      high + (size_t)colTile * d, d) * slack > rmax;
+  // The radii are fixed, so wave 0 computes the skip decisions of 64 walk
+  // steps at once (one barrier per 64 steps, none per skipped step).
+  const int steps = 2 * numTiles;
+  for (int base = 0; base < steps; base += 64) {
+    if (threadIdx.x < 64) {
+      const int step = base + (int)threadIdx.x;
+      const int colTile = step < steps ? walkStep(rowTile, step, numTiles) : -1;
+      skipStep[threadIdx.x] =
+          colTile < 0 || boxBound(metric, rowBox, rowBox + d, low + (size_t)colTile * d,
+                                  high + (size_t)colTile * d, d) * slack > rmax;
+    }
     __syncthreads();
-    const bool doSkip = skip;
-    __syncthreads();
-    if (doSkip)
-      continue;
-    ++computed;
-    const int col0 = colTile * T::DIM;
-    // Raw accumulators: only the pairs within the bound of the row's radius
-    // are finished (sqrt) and tested exactly.
-    float v[TR][TR];
-    computeTile<T, Metric, false>(sm, X, n, X, n, d, metric, row0, col0, v);
-#pragma unroll
-    for (int r = 0; r < TR; ++r) {
-      const int rit = T::rowInTile(r), row = row0 + rit;
-      if (row >= n)
+    const int batchEnd = min(steps, base + 64);
+    for (int step = base; step < batchEnd; ++step) {
+      if (skipStep[step - base])
         continue;
-      const float limit = metric.accBound(rRow[r]);
+      const int colTile = walkStep(rowTile, step, numTiles);
+      ++computed;
+      const int col0 = colTile * T::DIM;
+      // Raw accumulators: only the pairs within the bound of the row's radius
+      // are finished (sqrt) and tested exactly.
+      float v[TR][TR];
+      computeTile<T, Metric, false>(sm, X, n, X, n, d, metric, row0, col0, v);
 #pragma unroll
-      for (int c = 0; c < TR; ++c) {
-        const int col = col0 + T::colInTile(c);
-        if (col >= n || v[r][c] > limit)
+      for (int r = 0; r < TR; ++r) {
+        const int rit = T::rowInTile(r), row = row0 + rit;
+        if (row >= n)
           continue;
-        const float val = col == row ? 0.0f : fmaxf(metric.finish(v[r][c]), 0.0f);
-        if (val <= rRow[r]) {
-          const int pos = atomicAdd(&cnt[rit], 1);
-          if (pos < kCandPerRow)
-            cand[(size_t)row * kCandPerRow + pos] = val;
+        const float limit = metric.accBound(rRow[r]);
+#pragma unroll
+        for (int c = 0; c < TR; ++c) {
+          const int col = col0 + T::colInTile(c);
+          if (col >= n || v[r][c] > limit)
+            continue;
+          const float val = col == row ? 0.0f : fmaxf(metric.finish(v[r][c]), 0.0f);
+          if (val <= rRow[r]) {
+            const int pos = atomicAdd(&cnt[rit], 1);
+            if (pos < kCandPerRow)
+              cand[(size_t)row * kCandPerRow + pos] = val;
+          }
         }
       }
     }
+    __syncthreads(); // skipStep is rewritten by the next batch
   }
   __syncthreads();
   for (int i = threadIdx.x; i < T::DIM; i += T::THREADS)
