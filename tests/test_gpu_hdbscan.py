@@ -55,7 +55,7 @@ def test_core_methods_agree(k, metric):
 
 @pytest.mark.parametrize("n_features", [2, 8])
 @pytest.mark.parametrize("k", [1, 5, 40])
-@pytest.mark.parametrize("alpha", [1.0, 0.7])
+@pytest.mark.parametrize("alpha", [1.0, 0.7, 1.7])
 def test_pruned_tree_is_identical(n_features, k, alpha):
     X = blobs(n_samples=3000, n_features=n_features, centers=30, seed=20 + n_features)
     out = {}
