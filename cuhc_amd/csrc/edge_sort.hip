@@ -91,12 +91,13 @@ int treeFromDeviceEdges(const DeviceEdges &edges, int n, double *tree) {
     hipLaunchKernelGGL(edgeGatherKernel, grid, dim3(256), 0, stream(), edges.u.get(),
                        edges.v.get(), edges.w.get(), idxOut.get(), m, packed.get());
     s((int)hipGetLastError());
-    std::vector<PackedEdge> hp(m);
-    s(download(hp.data(), packed, m * sizeof(PackedEdge)));
+    s(downloadStaged(packed, m * sizeof(PackedEdge), [&](const void *p) {
+      const PackedEdge *hp = static_cast<const PackedEdge *>(p);
+      for (int i = 0; i < m; ++i)
+        host[i] = Edge{hp[i].u, hp[i].v, (double)hp[i].w};
+    }));
     if (!s.ok())
       return s.code;
-    for (int i = 0; i < m; ++i)
-      host[i] = Edge{hp[i].u, hp[i].v, (double)hp[i].w};
     // Runs of equal (weight, u): order by v.
     for (int i = 0; i < m;) {
       int j = i + 1;

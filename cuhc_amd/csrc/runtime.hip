@@ -1,5 +1,6 @@
 #include "runtime.hpp"
 
+#include <algorithm>
 #include <mutex>
 #include <unordered_map>
 
@@ -135,6 +136,55 @@ int download(void *dst, const void *src, size_t bytes) {
     return 0;
   int s = (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream());
   return s ? s : (int)hipStreamSynchronize(stream());
+}
+
+namespace {
+
+struct PinnedStage {
+  std::mutex mutex;
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+PinnedStage &pinnedStage() {
+  static PinnedStage *stage = new PinnedStage; // kept until the process ends
+  return *stage;
+}
+
+constexpr size_t kMaxPinnedStage = (size_t)256 << 20;
+
+} // namespace
+
+int downloadStaged(const void *src, size_t bytes,
+                   const std::function<void(const void *)> &consume) {
+  if (bytes == 0) {
+    consume(nullptr);
+    return 0;
+  }
+  PinnedStage &st = pinnedStage();
+  std::lock_guard<std::mutex> lock(st.mutex);
+  if (bytes <= kMaxPinnedStage && st.bytes < bytes) {
+    if (st.p)
+      (void)hipHostFree(st.p);
+    st.p = nullptr;
+    st.bytes = 0;
+    const size_t want = std::max(bytes, (size_t)1 << 20);
+    if (hipHostMalloc(&st.p, want, hipHostMallocDefault) == hipSuccess)
+      st.bytes = want;
+    else
+      st.p = nullptr;
+  }
+  if (st.bytes < bytes) {
+    std::vector<char> host(bytes);
+    const int s = download(host.data(), src, bytes);
+    if (s == 0)
+      consume(host.data());
+    return s;
+  }
+  const int s = download(st.p, src, bytes);
+  if (s == 0)
+    consume(st.p);
+  return s;
 }
 
 int fill(void *dst, int byte, size_t bytes) {

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
+#include <functional>
 #include <vector>
 
 namespace cuhc {
@@ -67,6 +68,10 @@ template <class T> struct DevArray {
 // Synchronous helpers on the library stream.
 int upload(void *dst, const void *src, size_t bytes);
 int download(void *dst, const void *src, size_t bytes);
+// Downloads bytes into a cached pinned host buffer (one DMA instead of the
+// staged chunks of a pageable copy) and hands it to consume while holding the
+// buffer; above 256 MiB, or if pinned memory is refused, a pageable copy.
+int downloadStaged(const void *src, size_t bytes, const std::function<void(const void *)> &consume);
 int fill(void *dst, int byte, size_t bytes);
 int sync();
 
