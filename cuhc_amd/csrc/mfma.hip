@@ -864,14 +864,36 @@ __global__ void componentRangesKernel(const int *__restrict__ sortedComp, int n,
     end[c] = i + 1;
 }
 
+// (size << 32 | component) of the largest component.
+__global__ void largestComponentKernel(const int *__restrict__ start, const int *__restrict__ end,
+                                       int numComp, unsigned long long *largest) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long key = 0;
+  if (c < numComp)
+    key = ((unsigned long long)(end[c] - start[c]) << 32) | (unsigned int)c;
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    const unsigned long long other = shflXorKey(key, m);
+    key = other > key ? other : key;
+  }
+  if ((threadIdx.x & 63) == 0 && key != 0)
+    atomicMax(largest, key);
+}
+
 // A row is settled when an entry of its kept list (the kTopK nearest outside
 // its component when it was last searched) lies in another component now: the
 // first such entry is its nearest there, written to nearest[]. Every other
-// row is searched again.
+// row is searched again, except the rows of a component holding more than
+// half of the points: that component proposes no edge this round (its
+// nearest[] entries are emptied). Boruvka only needs each proposed edge to be
+// its component's smallest outgoing one, and every other component still
+// proposes its own (the large component joins through theirs); its rows'
+// kept lists stay valid, as they hold the nearest points outside a subset of
+// the component.
 __global__ void staleRowsKernel(const int *__restrict__ perm, const int *__restrict__ comp,
                                 const unsigned long long *__restrict__ topk,
                                 unsigned long long *__restrict__ nearest, int n, bool firstRound,
-                                int *flag) {
+                                const unsigned long long *__restrict__ largest, int *flag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n)
     return;
@@ -883,6 +905,12 @@ __global__ void staleRowsKernel(const int *__restrict__ perm, const int *__restr
   bool stale = true;
   if (!firstRound) {
     const int own = comp[p];
+    const unsigned long long big = *largest;
+    if (2 * (long long)(big >> 32) > (long long)n && own == (int)(big & 0xffffffffu)) {
+      nearest[p] = kEmptyKey;
+      flag[i] = 0;
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < kTopK && stale; ++q) {
       const unsigned long long key = topk[(size_t)p * kTopK + q];
@@ -970,7 +998,7 @@ struct MixedPass {
   DevArray<unsigned char> sortTemp, scanTemp;
   size_t sortBytes = 0, scanBytes = 0;
   DevArray<uint2> cand;
-  DevArray<unsigned long long> stats, recheckStats, topk;
+  DevArray<unsigned long long> stats, recheckStats, topk, largest;
   float Rmax = 0.0f, rhoMax = 0.0f;
 };
 
@@ -1006,6 +1034,7 @@ MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
   s(mp->xp.alloc((size_t)n * d));
   s(mp->cand.alloc((size_t)2 * 2 * n * kCandCap));
   s(mp->topk.alloc((size_t)n * kTopK));
+  s(mp->largest.alloc(1));
   s(mp->stats.alloc(5));
   s(mp->recheckStats.alloc((size_t)kStatStripes * kStripeWords));
   s(mp->positions.alloc(n));
@@ -1103,11 +1132,16 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   hipLaunchKernelGGL(componentRangesKernel, grid, dim3(256), 0, stream(), mp->sortedComp.get(), n,
                      mp->compStart.get(), mp->compEnd.get());
   // Rows to search: all in the first round; later only points whose kept
-  // list has no entry outside their component (the others are settled here).
-  // Listed in component order.
+  // list has no entry outside their component (the others are settled here),
+  // and none of a component holding more than half of the points. Listed in
+  // component order.
+  s(fill(mp->largest, 0, sizeof(unsigned long long)));
+  hipLaunchKernelGGL(largestComponentKernel, dim3((numComponents + 255) / 256), dim3(256), 0,
+                     stream(), mp->compStart.get(), mp->compEnd.get(), numComponents,
+                     mp->largest.get());
   hipLaunchKernelGGL(staleRowsKernel, dim3((n + 1 + 255) / 256), dim3(256), 0, stream(),
                      mp->perm.get(), comp, mp->topk.get(), nearest, n, firstRound,
-                     mp->rowFlag.get());
+                     mp->largest.get(), mp->rowFlag.get());
   bytes = mp->scanBytes;
   s((int)hipcub::DeviceScan::ExclusiveSum(mp->scanTemp.get(), bytes, mp->rowFlag.get(),
                                           mp->rowIndex.get(), n + 1, stream()));
