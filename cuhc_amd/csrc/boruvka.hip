@@ -65,7 +65,7 @@ int boruvkaMst(const float *X, int n, int d, MetricSpec m, WeightSpec w, const M
   s(ct.alloc(n));
   if (!s.ok() || n < 2)
     return s.code;
-  DevArray<int> dComp, dPosOf;
+  DevArray<int> dComp, dPosOf, dSizes, dSkip;
   DevArray<unsigned int> dCompBest;
   DevArray<unsigned long long> dNearest;
   s(dComp.alloc(n));
@@ -75,9 +75,16 @@ int boruvkaMst(const float *X, int n, int d, MetricSpec m, WeightSpec w, const M
     if (s.ok())
       s(inverseOrder(opt.devOrig, n, dPosOf));
   }
-  // Pruned passes bound each component's search by its best known edge.
-  if (opt.prune)
+  // Pruned passes bound each component's search by its best known edge, and
+  // skip the rows of a component holding more than half of the points: it
+  // proposes no edge in that round and joins through the others' smallest
+  // edges (Boruvka only needs every proposed edge to be its component's
+  // smallest; every other component still proposes).
+  if (opt.prune) {
     s(dCompBest.alloc(n));
+    s(dSizes.alloc(n));
+    s(dSkip.alloc(1));
+  }
   if (s.ok())
     s(resetComponents(dComp, n));
   MixedPass *mixed = nullptr;
@@ -101,8 +108,9 @@ int boruvkaMst(const float *X, int n, int d, MetricSpec m, WeightSpec w, const M
         break;
       if (pruneThisPass) {
         double share = 1.0;
+        s(majorityComponent(dComp, n, numComponents, dSizes, dSkip));
         s(nearestOtherWalk(X, n, d, m, w, dComp, opt.devOrig, dCompBest, dNearest, &share,
-                           opt.stats));
+                           opt.stats, 1, dSkip));
         pruneThisPass = share <= kPruneGiveUpShare;
       } else {
         s(nearestOtherSymmetric(X, n, d, m, w, dComp, opt.devOrig, dNearest));

@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(256)
                            unsigned int *__restrict__ compBest,
                            unsigned long long *__restrict__ nearest,
                            unsigned long long *tilesComputed, int rowTileStride,
-                           int computeSteps) {
+                           int computeSteps, const int *__restrict__ skipComp) {
   constexpr int TR = T::TR;
   constexpr int kBatch = 64;
   __shared__ __align__(16) TileSmem<T> sm;
@@ -381,9 +381,14 @@ __global__ void __launch_bounds__(256)
 
   const int rowTile = blockIdx.x * rowTileStride, numTiles = boxes.numTiles;
   const int row0 = rowTile * T::DIM;
+  // Rows of the skipped component are closed; a tile of only such rows has
+  // nothing to do (uniform: every thread reads the same values).
+  const int skip = skipComp ? *skipComp : -1;
+  if (skip >= 0 && boxes.minComp[rowTile] == skip && boxes.maxComp[rowTile] == skip)
+    return;
   for (int i = threadIdx.x; i < T::DIM; i += T::THREADS) {
     rowBest[i] = kEmptyKey;
-    rowNeed[i] = row0 + i < n ? INFINITY : -1.0f;
+    rowNeed[i] = row0 + i < n && comp[row0 + i] != skip ? INFINITY : -1.0f;
   }
   for (int k = threadIdx.x; k < d; k += T::THREADS) {
     rowBox[k] = boxes.low[(size_t)rowTile * d + k];
@@ -392,7 +397,8 @@ __global__ void __launch_bounds__(256)
   // Wave 0 owns rows lane and lane + 64 for the bound: their components and
   // the components' best known edges (refreshed once per batch).
   const int ra = row0 + (int)(threadIdx.x & 63), rb = ra + 64;
-  const bool open0 = threadIdx.x < 64 && ra < n, open1 = threadIdx.x < 64 && rb < n;
+  const bool open0 = threadIdx.x < 64 && ra < n && comp[ra] != skip;
+  const bool open1 = threadIdx.x < 64 && rb < n && comp[rb] != skip;
   const int comp0 = open0 ? comp[ra] : 0, comp1 = open1 ? comp[rb] : 0;
   float cap0 = INFINITY, cap1 = INFINITY;
   auto capOf = [&](unsigned int bits) { return bits == 0xffffffffu ? INFINITY : floatOf(bits); };
@@ -410,7 +416,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int r = 0; r < TR; ++r) {
     const int row = row0 + T::rowInTile(r);
-    rowComp[r] = row < n ? comp[row] : -2;
+    rowComp[r] = row < n && comp[row] != skip ? comp[row] : -2;
     rowVal[r] = weight.value(row < n ? row : 0);
   }
   unsigned long long computed = 0;
@@ -824,7 +830,8 @@ constexpr int kEstimateSteps = 64;
 template <class T, class Metric, class Weight>
 int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &weight,
             const int *comp, const int *orig, unsigned int *compBest,
-            unsigned long long *nearest, double *share, PassStats *stats, int stride) {
+            unsigned long long *nearest, double *share, PassStats *stats, int stride,
+            const int *skipComp) {
   const int tiles = (n + T::DIM - 1) / T::DIM;
   DeviceBoxes boxes;
   DevArray<unsigned long long> counter;
@@ -843,7 +850,7 @@ int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &we
     hipLaunchKernelGGL((nearestOtherWalkKernel<T, Metric, Weight>), dim3(sampled), dim3(256),
                        2 * d * sizeof(float), stream(), X, n, d, metric, weight, comp, orig,
                        boxes.view, compBest, nearest, counter.get(), stride,
-                       stride > 1 ? kEstimateSteps : 0x7fffffff);
+                       stride > 1 ? kEstimateSteps : 0x7fffffff, skipComp);
     s((int)hipGetLastError());
   }
   unsigned long long computed = 0;
@@ -862,7 +869,7 @@ int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &we
 
 int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, const int *comp,
                      const int *orig, unsigned int *compBest, unsigned long long *nearest,
-                     double *share, PassStats *stats, int stride) {
+                     double *share, PassStats *stats, int stride, const int *skipComp) {
   if (!metricHasBoxBound(m.id))
     return kErrInvalidMetric;
   if (w.kind == kWeightWard) {
@@ -871,7 +878,7 @@ int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, c
     return withFill(d, [&](auto fill) {
       using T = Tile<8, decltype(fill)::value>;
       return runWalk<T>(X, n, d, SqEuclidean{}, WardWeight{w.size}, comp, orig, compBest,
-                        nearest, share, stats, stride);
+                        nearest, share, stats, stride, skipComp);
     });
   }
   return withMetric(m, [&](auto metric) {
@@ -883,7 +890,7 @@ int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, c
         return withFill(d, [&](auto fill) {
           using T = Tile<8, decltype(fill)::value>;
           return runWalk<T>(X, n, d, metric, weight, comp, orig, compBest, nearest, share,
-                            stats, stride);
+                            stats, stride, skipComp);
         });
       });
     }
@@ -907,6 +914,60 @@ int inverseOrder(const int *orig, int n, int *posOf) {
   hipLaunchKernelGGL(inverseOrderKernel, dim3((n + 255) / 256), dim3(256), 0, stream(), orig, n,
                      posOf);
   return (int)hipGetLastError();
+}
+
+namespace {
+
+// Component sizes; with few components each workgroup counts in LDS first
+// (same-address atomics from every point would serialise).
+constexpr int kSizeLds = 4096;
+
+__global__ void __launch_bounds__(256)
+    componentSizesKernel(const int *__restrict__ comp, int n, int numComp, int *sizes) {
+  __shared__ int local[kSizeLds];
+  const bool useLds = numComp <= kSizeLds;
+  if (useLds) {
+    for (int c = threadIdx.x; c < numComp; c += blockDim.x)
+      local[c] = 0;
+    __syncthreads();
+  }
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    if (useLds)
+      atomicAdd(&local[comp[p]], 1);
+    else
+      atomicAdd(&sizes[comp[p]], 1);
+  }
+  if (useLds) {
+    __syncthreads();
+    for (int c = threadIdx.x; c < numComp; c += blockDim.x)
+      if (local[c])
+        atomicAdd(&sizes[c], local[c]);
+  }
+}
+
+// At most one component holds more than half of the points.
+__global__ void majorityKernel(const int *__restrict__ sizes, int numComp, int n,
+                               int *majority) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < numComp && 2 * (long long)sizes[c] > (long long)n)
+    *majority = c;
+}
+
+} // namespace
+
+int majorityComponent(const int *comp, int n, int numComp, int *sizes, int *majority) {
+  Status s;
+  s(fill(majority, 0xff, sizeof(int))); // -1
+  if (2 * (long long)numComp > (long long)n + 1) // every component has at most n - numComp + 1 points
+    return s.code;
+  s(fill(sizes, 0, (size_t)numComp * sizeof(int)));
+  const int blocks = std::max(1, std::min(1024, (n + 255) / 256));
+  hipLaunchKernelGGL(componentSizesKernel, dim3(blocks), dim3(256), 0, stream(), comp, n, numComp,
+                     sizes);
+  hipLaunchKernelGGL(majorityKernel, dim3((numComp + 255) / 256), dim3(256), 0, stream(), sizes,
+                     numComp, n, majority);
+  s((int)hipGetLastError());
+  return s.code;
 }
 
 int estimatePrunedShare(const float *X, int n, int d, MetricSpec m, double *share) {

@@ -73,9 +73,16 @@ struct PassStats {
 // result is exact only when it holds its component's smallest edge, which is
 // all the contraction uses. Returns the share of tile pairs computed in
 // *share. rowTileStride > 1 runs every stride-th row tile only (sampling).
+// skipComp (nullable, device): rows of that component (-1: none) search
+// nothing and keep their nearest[] entries.
 int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, const int *comp,
                      const int *orig, unsigned int *compBest, unsigned long long *nearest,
-                     double *share, PassStats *stats, int rowTileStride = 1);
+                     double *share, PassStats *stats, int rowTileStride = 1,
+                     const int *skipComp = nullptr);
+
+// *majority (device) = the component holding more than half of the n points,
+// or -1. sizes: numComp ints of scratch.
+int majorityComponent(const int *comp, int n, int numComp, int *sizes, int *majority);
 
 // compBest[c] for c < numComp: "none", then (fromNearest) the weight bits of
 // the previous pass's candidates whose partner is still in another component.
