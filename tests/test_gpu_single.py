@@ -51,6 +51,23 @@ def test_mixed_on_clustered_and_offset_data():
     np.testing.assert_array_equal(run(Y, precision="mixed"), run(Y, precision="fp32"))
 
 
+@pytest.mark.parametrize("pruning", ["never", "always"])
+def test_majority_component_rounds(pruning):
+    # One dense cloud holding most points and a few small far groups: once the
+    # cloud holds more than half of the points it proposes no edge and joins
+    # through the small groups' smallest edges (mixed and pruned walk passes).
+    rng = np.random.default_rng(17)
+    big = rng.standard_normal((5000, 32)).astype(np.float32)
+    small = [rng.normal(loc=c, scale=0.3, size=(40, 32)) for c in (12.0, -15.0, 25.0)]
+    line = np.zeros((30, 32), np.float32)
+    line[:, 0] = 40.0 + np.arange(30, dtype=np.float32)  # a chain of equal gaps
+    X = np.concatenate([big, *small, line]).astype(np.float32)
+    X = X[rng.permutation(len(X))]
+    expected = oracle.single_linkage(X)
+    np.testing.assert_array_equal(run(X, precision="mixed"), expected)
+    np.testing.assert_array_equal(run(X, pruning=pruning), expected)
+
+
 def test_mixed_on_tied_grid():
     g = np.stack(np.meshgrid(np.arange(20), np.arange(20)), axis=-1).reshape(-1, 2)
     X = np.hstack([g, np.zeros((400, 14))]).astype(np.float32)
