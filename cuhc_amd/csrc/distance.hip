@@ -344,6 +344,44 @@ DEV int walkTile(int rowTile, int step, int numTiles) {
   return (tile >= 0 && tile < numTiles) ? tile : -1;
 }
 
+// Column data of a walk tile staged in LDS with the tile's first fill:
+// thread t < DIM loads column col0 + t's component, weight value and
+// original index (from a clamped index; -1 marks columns past n).
+template <class Weight> struct ColumnStage {
+  using WV = typename Weight::Value;
+  struct Value {
+    int comp;
+    WV val;
+    unsigned int idx;
+  };
+  const int *comp;
+  const int *orig;
+  Weight weight;
+  int col0, n, dim;
+  int *sComp;
+  WV *sVal;
+  unsigned int *sIdx;
+  DEV Value load() const {
+    Value v{};
+    if ((int)threadIdx.x < dim) {
+      const int col = col0 + (int)threadIdx.x;
+      const int cl = min(col, n - 1);
+      v.comp = comp[cl];
+      v.val = weight.value(cl);
+      v.idx = (unsigned)orig[cl];
+    }
+    return v;
+  }
+  DEV void store(const Value &v) const {
+    if ((int)threadIdx.x < dim) {
+      const bool ok = col0 + (int)threadIdx.x < n;
+      sComp[threadIdx.x] = ok ? v.comp : -1;
+      sVal[threadIdx.x] = v.val;
+      sIdx[threadIdx.x] = v.idx;
+    }
+  }
+};
+
 // One workgroup per row tile walks the column tiles outward from its own
 // (walkTile order). Lower bounds of the weight between the two boxes are
 // computed 64 steps at a time by wave 0, one step per lane, into LDS; a step
@@ -376,6 +414,9 @@ __global__ void __launch_bounds__(256)
   __shared__ float lowerBound[kBatch];
   __shared__ float bound;
   __shared__ int tileImproved; // a row's best improved in the current tile
+  __shared__ int sColComp[T::DIM];
+  __shared__ typename Weight::Value sColVal[T::DIM];
+  __shared__ unsigned int sColIdx[T::DIM];
   extern __shared__ float rowBox[]; // low[d], high[d]
   constexpr bool kFilter = Metric::kAccFilter && Weight::kDistBound;
 
@@ -491,16 +532,20 @@ __global__ void __launch_bounds__(256)
         continue; // estimate: counted against the bound reached so far, not computed
       const int col0 = colTile * T::DIM;
       float w[TR][TR]; // raw accumulators when filtering
-      computeTile<T, Metric, !kFilter>(sm, X, n, X, n, d, metric, row0, col0, w);
+      // The column data arrives with the tile's first fill (LDS, read after
+      // computeTile's barriers; rewritten only after this tile's last barrier).
+      const ColumnStage<Weight> stage{comp, orig, weight, col0, n, T::DIM,
+                                      sColComp, sColVal, sColIdx};
+      computeTile<T, Metric, !kFilter>(sm, X, n, X, n, d, metric, row0, col0, w, stage);
       int colComp[TR];
       typename Weight::Value colVal[TR];
       unsigned int colIdx[TR];
 #pragma unroll
       for (int c = 0; c < TR; ++c) {
-        const int col = col0 + T::colInTile(c);
-        colComp[c] = col < n ? comp[col] : -1;
-        colVal[c] = weight.value(col < n ? col : 0);
-        colIdx[c] = col < n ? (unsigned)orig[col] : 0u;
+        const int i = T::colInTile(c);
+        colComp[c] = sColComp[i];
+        colVal[c] = sColVal[i];
+        colIdx[c] = sColIdx[i];
       }
 #pragma unroll
       for (int r = 0; r < TR; ++r) {

@@ -75,14 +75,24 @@ DEV void stageSlice(float (*s)[T::LD], const float *__restrict__ pts, int n, int
   }
 }
 
+// No extra staging (see computeTile).
+struct NoExtraStage {
+  struct Value {};
+  DEV Value load() const { return {}; }
+  DEV void store(const Value &) const {}
+};
+
 // out[r][c] = metric(A[rowA0 + rowInTile(r)], B[colB0 + colInTile(c)]), or
 // the raw accumulator when kFinish is false (scalar accumulators only).
 // Entries outside the matrices are garbage. Contains __syncthreads(); every
-// thread of the workgroup must call it.
-template <class T, class Metric, bool kFinish = true>
+// thread of the workgroup must call it. extra.load() is issued with the first
+// fill's loads and extra.store() runs after that fill's LDS stores, before
+// its barrier: per-tile data the caller stages in LDS shares the fill's
+// memory round trip.
+template <class T, class Metric, bool kFinish = true, class Extra = NoExtraStage>
 DEV void computeTile(TileSmem<T> &sm, const float *__restrict__ A, int nA,
                      const float *__restrict__ B, int nB, int d, const Metric &metric, int rowA0,
-                     int colB0, float (&out)[T::TR][T::TR]) {
+                     int colB0, float (&out)[T::TR][T::TR], const Extra &extra = Extra{}) {
   constexpr int TR = T::TR;
   using Acc = typename Metric::Acc;
   Acc acc[TR][TR];
@@ -97,8 +107,13 @@ DEV void computeTile(TileSmem<T> &sm, const float *__restrict__ A, int nA,
     const int nf = min(T::FILL, d - f0);
     const int nf8 = (nf + 7) & ~7;
     __syncthreads(); // previous fill fully consumed
+    typename Extra::Value ev{};
+    if (f0 == 0)
+      ev = extra.load();
     stageSlice<T>(sm.a, A, nA, d, rowA0, f0, nf8);
     stageSlice<T>(sm.b, B, nB, d, colB0, f0, nf8);
+    if (f0 == 0)
+      extra.store(ev);
     __syncthreads();
     for (int k0 = 0; k0 < nf8; k0 += 8) {
 #pragma unroll
