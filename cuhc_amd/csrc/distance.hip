@@ -127,7 +127,10 @@ template <class Metric, class Weight> struct PairFilter {
   static constexpr bool kOn = (Metric::kAccFilter && Weight::kDistBound) || kWard;
   DEV static float limit(const Metric &metric, const Weight &weight,
                          const unsigned long long *nearest, int p) {
-    const unsigned int bits = reinterpret_cast<const unsigned int *>(nearest)[2 * (size_t)p + 1];
+    return limitOf(metric, weight,
+                   reinterpret_cast<const unsigned int *>(nearest)[2 * (size_t)p + 1]);
+  }
+  DEV static float limitOf(const Metric &metric, const Weight &weight, unsigned int bits) {
     if (bits == 0xffffffffu)
       return INFINITY;
     const float w = floatOf(bits);
@@ -141,6 +144,58 @@ template <class Metric, class Weight> struct PairFilter {
       return acc * fminf(vi, vj);
     else
       return acc;
+  }
+};
+
+// Row and column data of a symmetric tile staged in LDS with the tile's first
+// fill: threads [0, DIM) load column col0 + t, threads [DIM, 2 DIM) row
+// row0 + t - DIM (component, weight value, original index and, when
+// filtering, the weight bits of the current nearest, turned into a limit).
+// Points past n get component -1 (columns) or -2 (rows) and limit -inf.
+template <class Metric, class Weight, bool kFilter> struct PairStage {
+  using WV = typename Weight::Value;
+  struct Value {
+    int comp;
+    WV val;
+    unsigned int id, bits;
+  };
+  const int *comp;
+  const int *orig;
+  const unsigned long long *nearest;
+  Metric metric;
+  Weight weight;
+  int row0, col0, n, dim;
+  int *sComp;      // [2 dim]: columns, then rows
+  WV *sVal;        // [2 dim]
+  unsigned *sId;   // [2 dim]
+  float *sLimit;   // [2 dim]
+  DEV int point() const {
+    const int t = (int)threadIdx.x;
+    return t < dim ? col0 + t : row0 + (t - dim);
+  }
+  DEV Value load() const {
+    Value v{};
+    if ((int)threadIdx.x < 2 * dim) {
+      const int pl = min(point(), n - 1);
+      v.comp = comp[pl];
+      v.val = weight.value(pl);
+      v.id = orig ? (unsigned)orig[pl] : (unsigned)pl;
+      if (kFilter)
+        v.bits = reinterpret_cast<const unsigned int *>(nearest)[2 * (size_t)pl + 1];
+    }
+    return v;
+  }
+  DEV void store(const Value &v) const {
+    const int t = (int)threadIdx.x;
+    if (t < 2 * dim) {
+      const int p = point();
+      const bool ok = p < n;
+      sComp[t] = ok ? v.comp : (t < dim ? -1 : -2);
+      sVal[t] = v.val;
+      sId[t] = ok ? v.id : (unsigned)p;
+      if (kFilter)
+        sLimit[t] = ok ? PairFilter<Metric, Weight>::limitOf(metric, weight, v.bits) : -INFINITY;
+    }
   }
 };
 
@@ -160,6 +215,12 @@ __global__ void __launch_bounds__(256, 3) // three waves per SIMD (at most 168 V
   constexpr bool kFilter = Filter::kOn;
   __shared__ __align__(16) TileSmem<T> sm;
   __shared__ unsigned long long colBest[T::DIM];
+  // Point data of the tile: columns [0, DIM), rows [DIM, 2 DIM).
+  __shared__ int sComp[2 * T::DIM];
+  __shared__ typename Weight::Value sVal[2 * T::DIM];
+  __shared__ unsigned int sId[2 * T::DIM];
+  __shared__ float sLimit[2 * T::DIM];
+  static_assert(2 * T::DIM <= T::THREADS, "one point per thread");
 
   for (long long t = blockIdx.x; t < numPairs; t += gridDim.x) {
     int I, J;
@@ -169,17 +230,21 @@ __global__ void __launch_bounds__(256, 3) // three waves per SIMD (at most 168 V
       colBest[i] = kEmptyKey; // visible after computeTile's barriers
 
     float w[TR][TR]; // raw accumulators when filtering
-    computeTile<T, Metric, !kFilter>(sm, X, n, X, n, d, metric, row0, col0, w);
+    // The point data arrives with the tile's first fill (read after
+    // computeTile's barriers; rewritten only after this tile's last barrier).
+    const PairStage<Metric, Weight, kFilter> stage{comp, orig, nearest, metric, weight, row0,
+                                                   col0, n, T::DIM, sComp, sVal, sId, sLimit};
+    computeTile<T, Metric, !kFilter>(sm, X, n, X, n, d, metric, row0, col0, w, stage);
 
     int colComp[TR];
     unsigned int colId[TR];
     typename Weight::Value colVal[TR];
 #pragma unroll
     for (int c = 0; c < TR; ++c) {
-      const int col = col0 + T::colInTile(c);
-      colComp[c] = col < n ? comp[col] : -1;
-      colVal[c] = weight.value(col < n ? col : 0);
-      colId[c] = (unsigned)(orig && col < n ? orig[col] : col);
+      const int i = T::colInTile(c);
+      colComp[c] = sComp[i];
+      colVal[c] = sVal[i];
+      colId[c] = sId[i];
     }
     // A pair counts if both points exist, lie in different components and
     // have a finite weight (+inf never counts, as in the reference's
@@ -189,19 +254,17 @@ __global__ void __launch_bounds__(256, 3) // three waves per SIMD (at most 168 V
     float colLimit[TR];
     if constexpr (kFilter) {
 #pragma unroll
-      for (int c = 0; c < TR; ++c) {
-        const int col = col0 + T::colInTile(c);
-        colLimit[c] = col < n ? Filter::limit(metric, weight, nearest, col) : -INFINITY;
-      }
+      for (int c = 0; c < TR; ++c)
+        colLimit[c] = sLimit[T::colInTile(c)];
     }
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
-      const int row = row0 + T::rowInTile(r);
-      const int rc = row < n ? comp[row] : -2;
-      const typename Weight::Value rv = weight.value(row < n ? row : 0);
+      const int ri = T::DIM + T::rowInTile(r);
+      const int rc = sComp[ri];
+      const typename Weight::Value rv = sVal[ri];
       unsigned long long best = kEmptyKey;
       if constexpr (kFilter) {
-        const float rowLimit = row < n ? Filter::limit(metric, weight, nearest, row) : -INFINITY;
+        const float rowLimit = sLimit[ri];
 #pragma unroll
         for (int c = 0; c < TR; ++c) {
           const float acc = w[r][c];
@@ -244,10 +307,8 @@ __global__ void __launch_bounds__(256, 3) // three waves per SIMD (at most 168 V
       if (__ballot(countMask != 0) != 0) {
         unsigned int rowId[TR];
 #pragma unroll
-        for (int r = 0; r < TR; ++r) {
-          const int row = row0 + T::rowInTile(r);
-          rowId[r] = (unsigned)(orig && row < n ? orig[row] : row);
-        }
+        for (int r = 0; r < TR; ++r)
+          rowId[r] = sId[T::DIM + T::rowInTile(r)];
 #pragma unroll
         for (int c = 0; c < TR; ++c) {
           unsigned long long best = kEmptyKey;
