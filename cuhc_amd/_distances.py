@@ -1,13 +1,14 @@
 """Pairwise distances on the GPU and the input checks every entry point shares.
 
-Same surface and messages as reference python/cuhc/_distances.py.
+The names, metric ids, argument rules and error messages are the reference's
+interface (python/cuhc/_distances.py); the checks are written here.
 """
 
 import numpy as np
 
 from . import _lib
 
-# Metric names (sklearn's, with aliases) -> the C ABI metric id.
+# Metric names (scikit-learn's, with aliases) -> the C ABI metric id.
 METRIC_IDS = {
     "euclidean": 0, "l2": 0,
     "sqeuclidean": 1,
@@ -19,6 +20,18 @@ METRIC_IDS = {
 
 FLOAT32_MAX = float(np.finfo(np.float32).max)
 
+# Largest value a metric accumulates in FP32 before any final root, from the
+# per-feature spread s_k = max_k - min_k and magnitude a_k = max |x_k|:
+# |x_k - y_k| <= s_k, and cosine accumulates dot products and squared norms.
+_ACCUMULATED_BOUND = {
+    0: lambda s, a, p: (s * s).sum(),
+    1: lambda s, a, p: (s * s).sum(),
+    2: lambda s, a, p: s.sum(),
+    3: lambda s, a, p: s.max(),
+    4: lambda s, a, p: (s ** p).sum(),
+    5: lambda s, a, p: (a * a).sum(),
+}
+
 
 def validate_metric(metric, p):
     if metric not in METRIC_IDS:
@@ -28,44 +41,32 @@ def validate_metric(metric, p):
 
 
 def as_points(array, name):
-    """C-contiguous float32 (points x features), finite and non-empty."""
-    out = np.ascontiguousarray(array, dtype=np.float32)
-    if out.ndim != 2:
-        raise ValueError(f"{name} must be 2D (points x features), got shape {out.shape}")
-    if 0 in out.shape:
-        raise ValueError(f"{name} must be non-empty, got shape {out.shape}")
-    if not np.isfinite(out).all():
-        raise ValueError(f"{name} contains NaN or infinity")
-    return out
-
-
-def _distance_bound(spread, magnitude, metric, p):
-    kind = METRIC_IDS[metric]
-    if kind in (0, 1):
-        return float(np.sum(spread * spread))     # squared distances are accumulated
-    if kind == 2:
-        return float(np.sum(spread))
-    if kind == 3:
-        return float(np.max(spread))
-    if kind == 4:
-        return float(np.sum(spread ** p))         # sum of |x_k - y_k|^p before the root
-    return float(np.sum(magnitude * magnitude))   # cosine: dot products and squared norms
+    """The rows of `array` as a C-contiguous float32 (points x features) array,
+    checked to be 2-D, non-empty and finite."""
+    points = np.require(np.asarray(array, dtype=np.float32), requirements="C")
+    problem = None
+    if points.ndim != 2:
+        problem = f"must be 2D (points x features), got shape {points.shape}"
+    elif min(points.shape) == 0:
+        problem = f"must be non-empty, got shape {points.shape}"
+    elif not np.isfinite(points).all():
+        problem = "contains NaN or infinity"
+    if problem:
+        raise ValueError(f"{name} {problem}")
+    return points
 
 
 def check_distance_range(X, metric, p=2.0, Y=None, scale=1.0):
-    """Rejects inputs whose FP32 distances (times `scale`) could overflow.
-
-    |x_k - y_k| never exceeds feature k's spread, so the bound is exact per
-    feature; `scale` covers factors applied after the metric (Ward's cluster
-    sizes, HDBSCAN's 1 / alpha).
-    """
-    values = X if Y is None else np.vstack([X, Y])
+    """Raises ValueError when some FP32 distance (times `scale`: Ward's cluster
+    sizes, HDBSCAN's 1 / alpha) could overflow float32. The bound is exact per
+    feature, so data far from the origin passes as long as its spread fits."""
+    lo, hi = X.min(axis=0), X.max(axis=0)
+    if Y is not None:
+        lo, hi = np.minimum(lo, Y.min(axis=0)), np.maximum(hi, Y.max(axis=0))
+    lo, hi = lo.astype(np.float64), hi.astype(np.float64)
     with np.errstate(over="ignore", invalid="ignore"):
-        hi = values.max(axis=0).astype(np.float64)
-        lo = values.min(axis=0).astype(np.float64)
-        spread = hi - lo
-        magnitude = np.maximum(np.abs(hi), np.abs(lo))
-        bound = _distance_bound(spread, magnitude, metric, p) * scale
+        bound = float(_ACCUMULATED_BOUND[METRIC_IDS[metric]](
+            hi - lo, np.maximum(np.abs(lo), np.abs(hi)), p)) * scale
     if not bound <= FLOAT32_MAX:
         raise ValueError(
             f"the data's range is too large for float32 {metric} distances (they could reach "
@@ -86,17 +87,16 @@ def pairwise_distances(X, Y=None, metric="euclidean", p=2.0):
            [5., 0.]], dtype=float32)
     """
     validate_metric(metric, p)
-    X = as_points(X, "X")
-    if Y is not None:
-        Y = as_points(Y, "Y")
-        if Y.shape[1] != X.shape[1]:
-            raise ValueError(
-                f"X and Y must have the same number of features, got {X.shape[1]} and {Y.shape[1]}"
-            )
-    check_distance_range(X, metric, p, Y)
-    n_b = X.shape[0] if Y is None else Y.shape[0]
-    out = np.empty((X.shape[0], n_b), dtype=np.float32)
+    A = as_points(X, "X")
+    B = None if Y is None else as_points(Y, "Y")
+    if B is not None and B.shape[1] != A.shape[1]:
+        raise ValueError(
+            f"X and Y must have the same number of features, got {A.shape[1]} and {B.shape[1]}"
+        )
+    check_distance_range(A, metric, p, B)
+    cols = A if B is None else B
+    out = np.empty((len(A), len(cols)), dtype=np.float32)
     _lib.check(_lib.lib.cuhc_pairwise_distances(
-        _lib.float_ptr(X), X.shape[0], None if Y is None else _lib.float_ptr(Y), n_b,
-        X.shape[1], METRIC_IDS[metric], float(p), _lib.float_ptr(out)))
+        _lib.float_ptr(A), len(A), None if B is None else _lib.float_ptr(B), len(cols),
+        A.shape[1], METRIC_IDS[metric], float(p), _lib.float_ptr(out)))
     return out

@@ -1,6 +1,6 @@
 """scikit-learn compatible AgglomerativeClustering on the GPU.
 
-Same parameters, defaults, attributes and messages as reference
+Parameters, defaults, attributes and messages are those of reference
 python/cuhc/_estimators.py (and sklearn.cluster.AgglomerativeClustering).
 """
 
@@ -11,6 +11,19 @@ from sklearn.base import BaseEstimator, ClusterMixin
 
 from ._distances import as_points
 from ._linkage import LINKAGE_IDS, cut_tree, linkage, resolve_algorithm, validate_linkage
+
+
+def _argument_errors(est):
+    """The first problem with the estimator's parameters, as a message, or None."""
+    if (est.n_clusters is None) == (est.distance_threshold is None):
+        return ("Exactly one of n_clusters and distance_threshold has to be set, "
+                "and the other needs to be None.")
+    k = est.n_clusters
+    if k is not None and not (isinstance(k, numbers.Integral) and k >= 1):
+        return f"n_clusters must be a positive integer, got {k!r}"
+    if est.linkage not in LINKAGE_IDS:
+        return f"linkage must be one of {list(LINKAGE_IDS)}, got {est.linkage!r}"
+    return None
 
 
 class AgglomerativeClustering(ClusterMixin, BaseEstimator):
@@ -37,37 +50,28 @@ class AgglomerativeClustering(ClusterMixin, BaseEstimator):
         self.compute_distances = compute_distances
         self.algorithm = algorithm
 
-    def _validate_params(self):
-        if (self.n_clusters is None) == (self.distance_threshold is None):
-            raise ValueError(
-                "Exactly one of n_clusters and distance_threshold has to be set, "
-                "and the other needs to be None."
-            )
-        if self.n_clusters is not None and (
-                not isinstance(self.n_clusters, numbers.Integral) or self.n_clusters < 1):
-            raise ValueError(f"n_clusters must be a positive integer, got {self.n_clusters!r}")
-        if self.linkage not in LINKAGE_IDS:
-            raise ValueError(f"linkage must be one of {list(LINKAGE_IDS)}, got {self.linkage!r}")
+    def fit(self, X, y=None):
+        message = _argument_errors(self)
+        if message:
+            raise ValueError(message)
         validate_linkage(self.linkage, self.metric)
         resolve_algorithm(self.algorithm, self.linkage)
-
-    def fit(self, X, y=None):
-        self._validate_params()
-        X = as_points(X, "X")
-        n = X.shape[0]
+        points = as_points(X, "X")
+        n = len(points)
         if self.n_clusters is not None and self.n_clusters > n:
             raise ValueError(f"Cannot extract more clusters ({self.n_clusters}) than samples ({n})")
-        Z = linkage(X, method=self.linkage, metric=self.metric, algorithm=self.algorithm)
-        if self.distance_threshold is not None:
-            # sklearn's rule: merges at or above the threshold stay split.
-            self.n_clusters_ = int(np.count_nonzero(Z[:, 2] >= self.distance_threshold)) + 1
-        else:
-            self.n_clusters_ = int(self.n_clusters)
-        self.labels_ = cut_tree(Z, n - self.n_clusters_)
+        Z = linkage(points, method=self.linkage, metric=self.metric, algorithm=self.algorithm)
+        heights = Z[:, 2]
+        thresholded = self.distance_threshold is not None
+        # scikit-learn's threshold rule: merges at or above it are not applied.
+        k = int((heights >= self.distance_threshold).sum()) + 1 if thresholded \
+            else int(self.n_clusters)
+        self.n_clusters_ = k
+        self.labels_ = cut_tree(Z, n - k)
         self.children_ = Z[:, :2].astype(np.intp)
-        if self.compute_distances or self.distance_threshold is not None:
-            self.distances_ = Z[:, 2].copy()
+        if thresholded or self.compute_distances:
+            self.distances_ = heights.copy()
         self.n_leaves_ = n
-        self.n_features_in_ = X.shape[1]
+        self.n_features_in_ = points.shape[1]
         self.n_connected_components_ = 1
         return self

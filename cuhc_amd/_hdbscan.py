@@ -1,11 +1,13 @@
-"""HDBSCAN: GPU single-linkage tree of mutual reachability, then cluster extraction.
+"""HDBSCAN: the GPU single-linkage tree of mutual reachability, then cluster
+extraction in libcuhc's host code.
 
-The extraction reproduces sklearn's sklearn/cluster/_hdbscan/_tree.pyx
-(tree_to_labels: _condense_tree, _compute_stability, _get_clusters,
-_do_labelling, get_probabilities) result for result, as the reference does in
-python/cuhc/_hdbscan.py; condensing runs in C++ (cuhc_condense_tree). Written
-for n in the millions: children lists are precomputed, so selection is linear
-in the number of condensed clusters.
+The extraction gives scikit-learn's results (sklearn/cluster/_hdbscan/_tree.pyx,
+tree_to_labels), as the reference's python/cuhc/_hdbscan.py does: the
+condensed tree, stabilities, excess-of-mass or leaf selection, labels and
+membership probabilities run in C++ (cuhc_condense_tree, cuhc_cluster_stability,
+cuhc_select_clusters, cuhc_label_points, cuhc_membership_probabilities), each
+linear in the tree size. Only the cluster_selection_epsilon rule, which walks
+the small tree of clusters in scikit-learn's set order, runs here.
 """
 
 import numbers
@@ -18,309 +20,284 @@ from ._distances import METRIC_IDS, as_points, check_distance_range, validate_me
 
 NOISE = -1
 CORE_METHOD_IDS = {"auto": 0, "fused": 1, "radix": 2, "learned": 3}
+SELECTION_METHOD_IDS = {"eom": 0, "leaf": 1}
 
 
 def hdbscan_tree(X, min_samples, metric="euclidean", alpha=1.0, p=2.0, core_method="auto"):
     """(single-linkage tree of mutual reachability (n-1, 4), core distances (n,))."""
     X = as_points(X, "X")
     check_distance_range(X, metric, p, scale=1.0 / alpha)
-    n = X.shape[0]
+    n, d = X.shape
     tree = np.empty((max(n - 1, 0), 4), dtype=np.float64)
     core = np.empty(n, dtype=np.float32)
     _lib.check(_lib.lib.cuhc_hdbscan_tree(
-        _lib.float_ptr(X), n, X.shape[1], METRIC_IDS[metric], float(p), int(min_samples),
-        float(alpha), CORE_METHOD_IDS[core_method], _lib.double_ptr(tree), _lib.float_ptr(core)))
+        _lib.float_ptr(X), n, d, METRIC_IDS[metric], float(p), int(min_samples), float(alpha),
+        CORE_METHOD_IDS[core_method], _lib.double_ptr(tree), _lib.float_ptr(core)))
     return tree, core
 
 
 # ---------------------------------------------------------------------------
-# Condensed tree: rows (parent cluster, child point or cluster, lambda, size),
-# clusters numbered from n (the root), lambda = 1 / distance.
+# Condensed trees: four columns (parent cluster, child point or cluster,
+# lambda = 1 / distance, size); the root cluster is numbered n.
+
+
+class _Columns:
+    """A condensed tree as contiguous int64 / float64 columns for the C calls."""
+
+    def __init__(self, condensed):
+        parent, child, lam, size = condensed
+        self.parent = np.ascontiguousarray(parent, dtype=np.int64)
+        self.child = np.ascontiguousarray(child, dtype=np.int64)
+        self.lam = np.ascontiguousarray(lam, dtype=np.float64)
+        self.size = np.ascontiguousarray(size, dtype=np.int64)
+        self.rows = len(self.parent)
+        self.root = int(self.parent.min()) if self.rows else 0
+        self.top = int(self.parent.max()) if self.rows else -1
+
+    def args(self):
+        return (_lib.int64_ptr(self.parent), _lib.int64_ptr(self.child),
+                _lib.double_ptr(self.lam), _lib.int64_ptr(self.size), self.rows)
+
+    @property
+    def num_clusters(self):
+        return self.top - self.root + 1
 
 
 def condense_tree(Z, min_cluster_size):
-    """Condensed tree of a scipy-format single-linkage tree (C++ on the host)."""
+    """Condensed tree of a scipy-format single-linkage tree (C++ on the host),
+    rows in scikit-learn's _condense_tree order."""
     Z = np.ascontiguousarray(Z, dtype=np.float64)
     n = len(Z) + 1
-    capacity = n + 2 * (n // max(int(min_cluster_size), 1)) + 2
-    parent, child = np.empty(capacity, np.int64), np.empty(capacity, np.int64)
-    lam, size = np.empty(capacity, np.float64), np.empty(capacity, np.int64)
-    rows = _lib.lib.cuhc_condense_tree(
-        _lib.double_ptr(Z), n, int(min_cluster_size), capacity, _lib.int64_ptr(parent),
-        _lib.int64_ptr(child), _lib.double_ptr(lam), _lib.int64_ptr(size))
-    if rows < 0:
-        raise RuntimeError(f"condensed tree needs {-rows} rows, only {capacity} allocated")
-    return (parent[:rows].astype(np.intp), child[:rows].astype(np.intp), lam[:rows],
-            size[:rows].astype(np.intp))
+    room = n + 2 * (n // max(int(min_cluster_size), 1)) + 2
+    while True:
+        cols = (np.empty(room, np.int64), np.empty(room, np.int64), np.empty(room, np.float64),
+                np.empty(room, np.int64))
+        rows = _lib.lib.cuhc_condense_tree(
+            _lib.double_ptr(Z), n, int(min_cluster_size), room, _lib.int64_ptr(cols[0]),
+            _lib.int64_ptr(cols[1]), _lib.double_ptr(cols[2]), _lib.int64_ptr(cols[3]))
+        if rows >= 0:
+            break
+        room = -rows
+    return (cols[0][:rows].astype(np.intp), cols[1][:rows].astype(np.intp), cols[2][:rows],
+            cols[3][:rows].astype(np.intp))
 
 
-def condense_tree_numpy(Z, min_cluster_size):
-    """The same condensed tree computed level by level with NumPy (a cross-check of
-    the C++ version; same rows in the same order)."""
+def condense_tree_reference(Z, min_cluster_size):
+    """The condensed tree by a second, independent method (NumPy, one tree level
+    at a time); the same rows in the same order as condense_tree."""
     Z = np.asarray(Z, dtype=np.float64)
     n = len(Z) + 1
-    root = 2 * (n - 1)
     kids = Z[:, :2].astype(np.intp)
     height = Z[:, 2]
-    count = np.ones(2 * n - 1, dtype=np.intp)
-    count[n:] = Z[:, 3].astype(np.intp)
+    leaves = np.ones(2 * n - 1, dtype=np.intp)
+    leaves[n:] = Z[:, 3].astype(np.intp)
 
-    def subtree_bfs(start):
-        out, level = [], np.array([start], dtype=np.intp)
-        while len(level):
-            out.append(level)
-            inner = level[level >= n] - n
-            level = kids[inner].ravel()
-        return np.concatenate(out)
+    def points_below(node):
+        found, level = [], np.array([node], dtype=np.intp)
+        while level.size:
+            found.append(level[level < n])
+            level = kids[level[level >= n] - n].ravel()
+        return np.concatenate(found)
 
-    order = subtree_bfs(root)
-    label = np.zeros(root + 1, dtype=np.intp)
-    label[root] = n
-    nxt = n + 1
-    ignore = np.zeros(root + 1, dtype=bool)
-    rows = []
-    for node in order.tolist():
-        if node < n or ignore[node]:
-            continue
-        left, right = kids[node - n]
-        h = height[node - n]
-        lam = 1.0 / h if h > 0.0 else np.inf
-        lc, rc = count[left], count[right]
-        big_l, big_r = lc >= min_cluster_size, rc >= min_cluster_size
-        if big_l and big_r:
-            for side, c in ((left, lc), (right, rc)):
-                label[side] = nxt
-                nxt += 1
-                rows.append((np.array([label[node]]), np.array([label[side]]),
-                             np.array([lam]), np.array([c])))
-            continue
-        drops = []
-        if big_l and not big_r:
-            label[left] = label[node]
-            drops = [right]
-        elif big_r and not big_l:
-            label[right] = label[node]
-            drops = [left]
-        else:
-            drops = [left, right]
-        for side in drops:
-            sub = subtree_bfs(side)
-            ignore[sub] = True
-            pts = sub[sub < n]
-            rows.append((np.full(len(pts), label[node]), pts, np.full(len(pts), lam),
-                         np.ones(len(pts), dtype=np.intp)))
-    if not rows:
+    out = []
+    owner = {2 * n - 2: n}  # merge-tree node -> condensed cluster still growing
+    fresh = n + 1
+    level = [2 * n - 2]
+    while level:
+        deeper = []
+        for node in level:
+            if node < n or node not in owner:
+                continue
+            a, b = kids[node - n]
+            dist = height[node - n]
+            lam = np.inf if dist <= 0.0 else 1.0 / dist
+            big = [leaves[a] >= min_cluster_size, leaves[b] >= min_cluster_size]
+            me = owner[node]
+            for side, keep in zip((a, b), big):
+                if all(big):
+                    owner[side] = fresh
+                    out.append(([me], [fresh], [lam], [leaves[side]]))
+                    fresh += 1
+                elif keep:
+                    owner[side] = me
+                else:
+                    pts = points_below(side)
+                    out.append((np.full(pts.size, me), pts, np.full(pts.size, lam),
+                                np.ones(pts.size, dtype=np.intp)))
+            deeper.extend((a, b))
+        level = deeper
+    if not out:
         empty = np.empty(0, dtype=np.intp)
-        return empty, empty, np.empty(0), empty
-    cols = [np.concatenate([r[i] for r in rows]) for i in range(4)]
-    return (cols[0].astype(np.intp), cols[1].astype(np.intp), cols[2].astype(np.float64),
-            cols[3].astype(np.intp))
+        return empty, empty.copy(), np.empty(0), empty.copy()
+    return tuple(np.concatenate([np.asarray(part[i]) for part in out]).astype(dtype)
+                 for i, dtype in enumerate((np.intp, np.intp, np.float64, np.intp)))
+
+
+def _stability_array(cols):
+    out = np.zeros(max(cols.num_clusters, 0))
+    _lib.check(_lib.lib.cuhc_cluster_stability(*cols.args(), _lib.double_ptr(out)))
+    return out
 
 
 def compute_stability(condensed):
-    """{cluster: sum over its rows of (lambda - lambda_birth) * size}, summed in row
-    order (the same floating-point order as sklearn)."""
-    parents, children, lambdas, sizes = condensed
-    smallest = int(parents.min())
-    largest = max(int(children.max()), smallest)
-    birth = np.full(largest + 1, np.nan)
-    birth[children] = lambdas
-    birth[smallest] = 0.0
-    totals = np.zeros(int(parents.max()) - smallest + 1)
-    np.add.at(totals, parents - smallest, (lambdas - birth[parents]) * sizes)
-    return {smallest + i: float(v) for i, v in enumerate(totals)}
+    """{cluster id: stability}: the sum over the cluster's rows, in row order, of
+    (lambda - the cluster's birth lambda) * size."""
+    cols = _Columns(condensed)
+    return dict(zip(range(cols.root, cols.top + 1), _stability_array(cols).tolist()))
 
 
-def _death_lambdas(condensed):
-    """Per parent: the largest lambda of the last run of consecutive rows with that
-    parent (sklearn's max_lambdas, quirk included)."""
-    parents, _, lambdas, _ = condensed
-    deaths = np.zeros(int(parents.max()) + 1)
-    starts = np.concatenate([[0], np.flatnonzero(parents[1:] != parents[:-1]) + 1])
-    run_max = np.maximum.reduceat(lambdas, starts)
-    run_parent = parents[starts]
-    # Later runs overwrite earlier ones: keep the last run of each parent.
-    _, last = np.unique(run_parent[::-1], return_index=True)
-    keep = len(starts) - 1 - last
-    deaths[run_parent[keep]] = run_max[keep]
-    return deaths
+def _epsilon_rule(cols, candidates, epsilon, allow_single_cluster):
+    """scikit-learn's epsilon search over the tree of clusters: a candidate that
+    split off below distance epsilon is replaced by its nearest ancestor that
+    split off above it (the root's child next to the root, or the root when a
+    single cluster is allowed). Candidates are visited in the iteration order of
+    the set scikit-learn builds from the same list, which decides overlaps."""
+    is_cluster_row = cols.size > 1
+    up = dict(zip(cols.child[is_cluster_row].tolist(), cols.parent[is_cluster_row].tolist()))
+    born = dict(zip(cols.child[is_cluster_row].tolist(), cols.lam[is_cluster_row].tolist()))
+    below = {}
+    for c, p in up.items():
+        below.setdefault(p, []).append(c)
 
-
-class _ClusterTree:
-    """The cluster rows (size > 1) of a condensed tree with children lists."""
-
-    def __init__(self, condensed):
-        parents, children, lambdas, sizes = condensed
-        mask = sizes > 1
-        self.parents = parents[mask]
-        self.children = children[mask]
-        self.lambdas = lambdas[mask]
-        self.sizes = sizes[mask]
-        self.kids = {}
-        for p, c in zip(self.parents.tolist(), self.children.tolist()):
-            self.kids.setdefault(p, []).append(c)
-        self.parent_of = dict(zip(self.children.tolist(), self.parents.tolist()))
-        self.lambda_of = dict(zip(self.children.tolist(), self.lambdas.tolist()))
-        self.root = int(self.parents.min()) if len(self.parents) else None
-
-    def bfs(self, start):
-        out, level = [], [start]
-        while level:
-            out.extend(level)
-            level = [c for node in level for c in self.kids.get(node, ())]
-        return out
-
-    def leaves(self):
-        if self.root is None:
-            return []
-        out, stack = [], [self.root]
-        while stack:  # depth first, children in row order (sklearn's recursion)
-            node = stack.pop()
-            kids = self.kids.get(node)
-            if not kids:
-                out.append(node)
-            else:
-                stack.extend(reversed(kids))
-        return out
-
-    def climb(self, leaf, epsilon, allow_single_cluster):
-        """sklearn's traverse_upwards."""
+    def climb(c):
         while True:
-            parent = self.parent_of[leaf]
-            if parent == self.root:
-                return parent if allow_single_cluster else leaf
-            if 1.0 / self.lambda_of[parent] > epsilon:
-                return parent
-            leaf = parent
+            p = up[c]
+            if p == cols.root:
+                return p if allow_single_cluster else c
+            if 1.0 / born[p] > epsilon:
+                return p
+            c = p
 
-    def epsilon_search(self, leaves, epsilon, allow_single_cluster):
-        selected, processed = [], set()
-        for leaf in leaves:
-            if 1.0 / self.lambda_of[leaf] < epsilon:
-                if leaf not in processed:
-                    top = self.climb(leaf, epsilon, allow_single_cluster)
-                    selected.append(top)
-                    processed.update(c for c in self.bfs(top) if c != top)
-            else:
-                selected.append(leaf)
-        return set(selected)
+    def subtree(c):
+        stack, out = [c], []
+        while stack:
+            x = stack.pop()
+            out.append(x)
+            stack.extend(below.get(x, ()))
+        return out
+
+    chosen, covered = [], set()
+    for c in set(candidates):
+        if 1.0 / born[c] >= epsilon:
+            chosen.append(c)
+        elif c not in covered:
+            top = climb(c)
+            chosen.append(top)
+            covered.update(x for x in subtree(top) if x != top)
+    return set(chosen)
+
+
+def _cluster_tree_leaves(cols):
+    """Leaves of the tree of clusters in depth-first order (children in row order)."""
+    rows = cols.size > 1
+    if not rows.any():
+        return []
+    below = {}
+    for p, c in zip(cols.parent[rows].tolist(), cols.child[rows].tolist()):
+        below.setdefault(p, []).append(c)
+    out, stack = [], [cols.root]
+    while stack:
+        c = stack.pop()
+        kids = below.get(c)
+        if kids:
+            stack.extend(reversed(kids))
+        else:
+            out.append(c)
+    return out
+
+
+def _select(cols, stab, method, allow_single_cluster, epsilon, max_cluster_size):
+    if cols.rows == 0:
+        return set()
+    stab = np.array(stab, dtype=np.float64)  # updated in place by excess of mass
+    chosen = np.zeros(cols.num_clusters, dtype=np.uint8)
+    _lib.check(_lib.lib.cuhc_select_clusters(
+        *cols.args(), _lib.double_ptr(stab), SELECTION_METHOD_IDS[method],
+        int(bool(allow_single_cluster)), -1 if max_cluster_size is None else int(max_cluster_size),
+        chosen.ctypes.data_as(_lib._u8p)))
+    picked = (np.flatnonzero(chosen) + cols.root).tolist()
+    if epsilon == 0.0 or not (cols.size > 1).any():
+        return set(picked)
+    # The epsilon rule starts from the candidates in scikit-learn's list order:
+    # descending ids for excess of mass, depth-first leaves for leaf selection.
+    first = cols.root if allow_single_cluster else cols.root + 1
+    if method == "eom":
+        candidates = sorted(picked, reverse=True)
+        if candidates == [cols.root]:
+            return set(candidates) if allow_single_cluster else set()
+    else:
+        candidates = _cluster_tree_leaves(cols)
+    chosen_eps = _epsilon_rule(cols, candidates, epsilon, allow_single_cluster)
+    return {c for c in chosen_eps if first <= c <= cols.top}
 
 
 def select_clusters(condensed, stability, cluster_selection_method="eom",
                     allow_single_cluster=False, cluster_selection_epsilon=0.0,
                     max_cluster_size=None):
-    """The selected cluster ids (sklearn's _get_clusters without the labelling)."""
-    parents, children, _, sizes = condensed
-    stability = dict(stability)
-    nodes = sorted(stability, reverse=True)  # children have larger ids: bottom-up
-    if not allow_single_cluster:
-        nodes = nodes[:-1]
-    tree = _ClusterTree(condensed)
-    is_cluster = {c: True for c in nodes}
-    n_samples = int(children[sizes == 1].max()) + 1
-    if max_cluster_size is None:
-        max_cluster_size = n_samples + 1
-    cluster_sizes = dict(zip(tree.children.tolist(), tree.sizes.tolist()))
-    if allow_single_cluster:
-        cluster_sizes[nodes[-1]] = int(tree.sizes[tree.parents == nodes[-1]].sum())
+    """The selected cluster ids (scikit-learn's _get_clusters before labelling);
+    stability is {cluster id: value} as compute_stability returns it."""
+    cols = _Columns(condensed)
+    stab = [stability[c] for c in range(cols.root, cols.top + 1)]
+    return _select(cols, stab, cluster_selection_method, allow_single_cluster,
+                   cluster_selection_epsilon, max_cluster_size)
 
-    if cluster_selection_method == "eom":
-        for node in nodes:
-            kids = tree.kids.get(node, [])
-            below = float(np.sum([stability[c] for c in kids]))
-            if below > stability[node] or cluster_sizes[node] > max_cluster_size:
-                is_cluster[node] = False
-                stability[node] = below
-            else:
-                for sub in tree.bfs(node):
-                    if sub != node:
-                        is_cluster[sub] = False
-        if cluster_selection_epsilon != 0.0 and len(tree.children) > 0:
-            eom = [c for c in is_cluster if is_cluster[c]]
-            chosen = []
-            if len(eom) == 1 and eom[0] == tree.root:
-                if allow_single_cluster:
-                    chosen = eom
-            else:
-                chosen = tree.epsilon_search(set(eom), cluster_selection_epsilon,
-                                             allow_single_cluster)
-            for c in is_cluster:
-                is_cluster[c] = c in chosen
-    elif cluster_selection_method == "leaf":
-        leaves = set(tree.leaves())
-        if not leaves:
-            for c in is_cluster:
-                is_cluster[c] = False
-            is_cluster[int(parents.min())] = True
-        if cluster_selection_epsilon != 0.0:
-            chosen = tree.epsilon_search(leaves, cluster_selection_epsilon, allow_single_cluster)
-        else:
-            chosen = leaves
-        for c in is_cluster:
-            is_cluster[c] = c in chosen
-    return {c for c in is_cluster if is_cluster[c]}
+
+def _selected_mask(cols, clusters):
+    mask = np.zeros(max(cols.num_clusters, 0), dtype=np.uint8)
+    for c in clusters:
+        mask[c - cols.root] = 1
+    return mask
 
 
 def label_points(condensed, clusters, allow_single_cluster, cluster_selection_epsilon):
-    """Label of every point: its nearest selected ancestor cluster (labels in sorted
-    cluster order), else noise. Equivalent to sklearn's union-find labelling."""
-    parents, children, lambdas, sizes = condensed
-    root = int(parents.min())
-    n = root
-    label_of = {c: i for i, c in enumerate(sorted(clusters))}
-    owner = {root: root}
-    for p, c in zip(parents[sizes > 1].tolist(), children[sizes > 1].tolist()):
-        owner[c] = c if c in clusters else owner[p]
-    labels = np.full(n, NOISE, dtype=np.intp)
-    point_rows = np.flatnonzero(children < n)
-    point_parents = parents[point_rows]
-    uniq, inv = np.unique(point_parents, return_inverse=True)
-    owner_of_row = np.array([owner[p] for p in uniq.tolist()], dtype=np.intp)[inv]
-    inside = owner_of_row != root
-    labels[children[point_rows[inside]]] = np.array(
-        [label_of[c] for c in owner_of_row[inside].tolist()], dtype=np.intp)
-    if len(clusters) == 1 and allow_single_cluster and root in clusters:
-        if cluster_selection_epsilon != 0.0:
-            threshold = 1.0 / cluster_selection_epsilon
-        else:
-            threshold = lambdas[parents == root].max()
-        at_root = point_rows[~inside]
-        keep = lambdas[at_root] >= threshold
-        labels[children[at_root[keep]]] = label_of[root]
-    return labels
+    """Label of every point: the rank of its selected cluster among the sorted
+    selected ids, or -1 (noise)."""
+    cols = _Columns(condensed)
+    labels = np.empty(cols.root, dtype=np.int64)
+    _lib.check(_lib.lib.cuhc_label_points(
+        *cols.args(), _selected_mask(cols, clusters).ctypes.data_as(_lib._u8p),
+        int(bool(allow_single_cluster)), float(cluster_selection_epsilon),
+        _lib.int64_ptr(labels)))
+    return labels.astype(np.intp)
 
 
 def membership_probabilities(condensed, clusters, labels):
-    """min(lambda_point, lambda_max) / lambda_max of the point's cluster (1 when
-    lambda_max is 0 or the point's lambda is infinite), 0 for noise."""
-    parents, children, lambdas, _ = condensed
-    deaths = _death_lambdas(condensed)
-    ordered = np.array(sorted(clusters), dtype=np.intp)
-    out = np.zeros(len(labels))
-    rows = np.flatnonzero(children < parents.min())
-    pts = children[rows]
-    labelled = labels[pts] != NOISE
-    rows, pts = rows[labelled], pts[labelled]
-    if len(rows) == 0:
-        return out
-    top = deaths[ordered[labels[pts]]]
-    lam = lambdas[rows]
-    certain = (top == 0.0) | np.isinf(lam)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        ratio = np.minimum(lam, top) / top
-    out[pts] = np.where(certain, 1.0, ratio)
+    """Membership strength of each point in its cluster, 0 for noise."""
+    cols = _Columns(condensed)
+    labels = np.ascontiguousarray(labels, dtype=np.int64)
+    out = np.zeros(cols.root, dtype=np.float64)
+    _lib.check(_lib.lib.cuhc_membership_probabilities(
+        *cols.args(), _selected_mask(cols, clusters).ctypes.data_as(_lib._u8p),
+        _lib.int64_ptr(labels), _lib.double_ptr(out)))
     return out
 
 
 def tree_to_labels(Z, min_cluster_size, cluster_selection_method="eom",
                    allow_single_cluster=False, cluster_selection_epsilon=0.0,
                    max_cluster_size=None):
-    """(labels, probabilities) from a single-linkage tree, as sklearn's tree_to_labels."""
-    condensed = condense_tree(Z, min_cluster_size)
-    clusters = select_clusters(condensed, compute_stability(condensed), cluster_selection_method,
-                               allow_single_cluster, cluster_selection_epsilon, max_cluster_size)
-    labels = label_points(condensed, clusters, allow_single_cluster, cluster_selection_epsilon)
-    return labels, membership_probabilities(condensed, clusters, labels)
+    """(labels, probabilities) from a single-linkage tree, as scikit-learn's
+    tree_to_labels."""
+    cols = _Columns(condense_tree(Z, min_cluster_size))
+    clusters = _select(cols, _stability_array(cols), cluster_selection_method,
+                       allow_single_cluster, cluster_selection_epsilon, max_cluster_size)
+    mask = _selected_mask(cols, clusters).ctypes.data_as(_lib._u8p)
+    labels = np.empty(cols.root, dtype=np.int64)
+    probabilities = np.zeros(cols.root, dtype=np.float64)
+    _lib.check(_lib.lib.cuhc_label_points(
+        *cols.args(), mask, int(bool(allow_single_cluster)), float(cluster_selection_epsilon),
+        _lib.int64_ptr(labels)))
+    _lib.check(_lib.lib.cuhc_membership_probabilities(
+        *cols.args(), mask, _lib.int64_ptr(labels), _lib.double_ptr(probabilities)))
+    return labels.astype(np.intp), probabilities
+
+
+def _require(ok, message):
+    if not ok:
+        raise ValueError(message)
+
+
+def _is_int(value, lowest):
+    return isinstance(value, numbers.Integral) and value >= lowest
 
 
 class HDBSCAN(ClusterMixin, BaseEstimator):
@@ -352,36 +329,27 @@ class HDBSCAN(ClusterMixin, BaseEstimator):
         self.cluster_selection_method = cluster_selection_method
         self.allow_single_cluster = allow_single_cluster
 
-    def _validate_params(self):
-        def integer_at_least(value, lowest):
-            return isinstance(value, numbers.Integral) and value >= lowest
-
-        if not integer_at_least(self.min_cluster_size, 2):
-            raise ValueError(
-                f"min_cluster_size must be an integer >= 2, got {self.min_cluster_size!r}")
-        if self.min_samples is not None and not integer_at_least(self.min_samples, 1):
-            raise ValueError(
-                f"min_samples must be None or an integer >= 1, got {self.min_samples!r}")
-        if self.max_cluster_size is not None and not integer_at_least(self.max_cluster_size, 1):
-            raise ValueError(
-                f"max_cluster_size must be None or an integer >= 1, got {self.max_cluster_size!r}")
-        if not self.cluster_selection_epsilon >= 0:
-            raise ValueError("cluster_selection_epsilon must be >= 0, got "
-                             f"{self.cluster_selection_epsilon!r}")
-        if not self.alpha > 0:
-            raise ValueError(f"alpha must be > 0, got {self.alpha!r}")
-        if self.cluster_selection_method not in ("eom", "leaf"):
-            raise ValueError("cluster_selection_method must be 'eom' or 'leaf', got "
-                             f"{self.cluster_selection_method!r}")
+    def _check_parameters(self):
+        mcs, ms, top = self.min_cluster_size, self.min_samples, self.max_cluster_size
+        _require(_is_int(mcs, 2), f"min_cluster_size must be an integer >= 2, got {mcs!r}")
+        _require(ms is None or _is_int(ms, 1),
+                 f"min_samples must be None or an integer >= 1, got {ms!r}")
+        _require(top is None or _is_int(top, 1),
+                 f"max_cluster_size must be None or an integer >= 1, got {top!r}")
+        _require(self.cluster_selection_epsilon >= 0,
+                 f"cluster_selection_epsilon must be >= 0, got {self.cluster_selection_epsilon!r}")
+        _require(self.alpha > 0, f"alpha must be > 0, got {self.alpha!r}")
+        _require(self.cluster_selection_method in SELECTION_METHOD_IDS,
+                 "cluster_selection_method must be 'eom' or 'leaf', got "
+                 f"{self.cluster_selection_method!r}")
         validate_metric(self.metric, 2.0)
 
     def fit(self, X, y=None):
-        self._validate_params()
+        self._check_parameters()
         X = as_points(X, "X")
-        n = X.shape[0]
-        k = self.min_cluster_size if self.min_samples is None else self.min_samples
-        if k > n:
-            raise ValueError(f"min_samples ({k}) must be at most the number of samples in X ({n})")
+        k = self.min_samples if self.min_samples is not None else self.min_cluster_size
+        _require(k <= len(X),
+                 f"min_samples ({k}) must be at most the number of samples in X ({len(X)})")
         Z, core = hdbscan_tree(X, k, metric=self.metric, alpha=self.alpha)
         self.labels_, self.probabilities_ = tree_to_labels(
             Z, self.min_cluster_size, self.cluster_selection_method, self.allow_single_cluster,

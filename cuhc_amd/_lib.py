@@ -44,6 +44,12 @@ _declare("cuhc_prune_stats", None, _u64p, _u64p)
 _declare("cuhc_error_string", ctypes.c_char_p, _int)
 
 # Extensions (no counterpart in the reference library).
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_CONDENSED = (_i64p, _i64p, _f64p, _i64p, _i64)
+_declare("cuhc_cluster_stability", _int, *_CONDENSED, _f64p)
+_declare("cuhc_select_clusters", _int, *_CONDENSED, _f64p, _int, _int, _i64, _u8p)
+_declare("cuhc_label_points", _int, *_CONDENSED, _u8p, _int, ctypes.c_double, _i64p)
+_declare("cuhc_membership_probabilities", _int, *_CONDENSED, _u8p, _i64p, _f64p)
 _declare("cuhc_set_precision", _int, _int)
 _declare("cuhc_linkage_device", _int, ctypes.c_void_p, _int, _int, _int, _float, _int, _int,
          _f64p)
@@ -57,6 +63,7 @@ _declare("cuhc_copy_to_host", _int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_u
 _declare("cuhc_set_timing", _int, _int)
 _declare("cuhc_read_timings", None, _f64p, _i64p, _i64p, _int)
 _declare("cuhc_reset_timings", None)
+_declare("cuhc_release", None)
 _declare("cuhc_mixed_stats", None, _u64p, _u64p, _u64p, _u64p)
 _declare("cuhc_learned_stats", None, _u64p, _u64p, _u64p, _u64p, _u64p)
 _declare("cuhc_set_learned_training", _int, _int)
@@ -132,8 +139,8 @@ def read_timings():
 
 
 class DeviceArray:
-    """A host array copied into HBM through the library's own HIP runtime (no
-    torch: its bundled HIP runtime cannot share the process with this one)."""
+    """A host array copied into HBM with the library's allocation and copy entry
+    points; .ptr is the device address the *_device entry points take."""
 
     def __init__(self, host):
         import numpy as np

@@ -1,18 +1,23 @@
 """Merge trees in scipy's linkage format and flat cuts of them.
 
-Same surface, algorithm choices and messages as reference python/cuhc/_linkage.py.
+The names, ids, algorithm rules and error messages are the reference's
+interface (python/cuhc/_linkage.py); the code is written here.
 """
 
 import numpy as np
+from scipy.sparse import coo_matrix
+from scipy.sparse.csgraph import connected_components
 
 from . import _lib
 from ._distances import METRIC_IDS, as_points, check_distance_range, validate_metric
 
 LINKAGE_IDS = {"single": 0, "complete": 1, "average": 2, "ward": 3}
 ALGORITHM_IDS = {"dense": 0, "mst": 1, "nn_chain": 2, "rnn": 3}
+# Which linkages each algorithm builds, and the one "auto" picks per linkage.
 ALGORITHM_LINKAGES = {
     "dense": set(LINKAGE_IDS), "mst": {"single"}, "nn_chain": {"ward"}, "rnn": {"ward"},
 }
+_AUTO = {"single": "mst", "ward": "rnn", "complete": "dense", "average": "dense"}
 # Linkages defined for euclidean distances only.
 EUCLIDEAN_ONLY = {"ward"}
 
@@ -27,16 +32,15 @@ def validate_linkage(method, metric):
 def resolve_algorithm(algorithm, method, num_points=None):
     """The algorithm to run; "auto" is mst for single, rnn for ward, dense otherwise."""
     if algorithm == "auto":
-        return {"single": "mst", "ward": "rnn"}.get(method, "dense")
+        return _AUTO[method]
     if algorithm not in ALGORITHM_IDS:
         raise ValueError(
             f"unknown algorithm {algorithm!r}; expected 'auto' or one of {list(ALGORITHM_IDS)}"
         )
-    if method not in ALGORITHM_LINKAGES[algorithm]:
+    supported = ALGORITHM_LINKAGES[algorithm]
+    if method not in supported:
         raise ValueError(
-            f"algorithm={algorithm!r} supports linkage {sorted(ALGORITHM_LINKAGES[algorithm])}, "
-            f"not {method!r}"
-        )
+            f"algorithm={algorithm!r} supports linkage {sorted(supported)}, not {method!r}")
     return algorithm
 
 
@@ -56,60 +60,64 @@ def linkage(X, method="single", metric="euclidean", p=2.0, *, algorithm="auto"):
            [2., 3., 2., 2.],
            [4., 5., 4., 4.]])
     """
+    # Arguments are checked before the points are converted.
     validate_linkage(method, metric)
-    resolve_algorithm(algorithm, method)
+    chosen = resolve_algorithm(algorithm, method)
     validate_metric(metric, p)
-    X = as_points(X, "X")
-    n = X.shape[0]
-    # Ward distances grow with cluster size: 2 n_a n_b / (n_a + n_b) <= n.
-    check_distance_range(X, metric, p, scale=n if method == "ward" else 1.0)
-    algorithm = resolve_algorithm(algorithm, method, n)
-    tree = np.empty((max(n - 1, 0), 4), dtype=np.float64)
-    if n == 1:
-        return tree
-    status = _lib.lib.cuhc_linkage(
-        _lib.float_ptr(X), n, X.shape[1], METRIC_IDS[metric], float(p), LINKAGE_IDS[method],
-        ALGORITHM_IDS[algorithm], _lib.double_ptr(tree))
-    if status == _lib.HIP_ERROR_OUT_OF_MEMORY and algorithm == "dense":
-        hint = " (ward linkage can use algorithm='rnn', which doesn't need it)" \
-            if method == "ward" else ""
-        raise MemoryError(
-            f"cuhc.linkage: the dense distance matrix for {n} points needs "
-            f"{2 * n * (n - 1) / 1e9:.1f} GB of GPU memory, which isn't available{hint}")
+    points = as_points(X, "X")
+    n, d = points.shape
+    # Ward's squared heights reach 2 a b / (a + b) <= n times a squared point distance.
+    check_distance_range(points, metric, p, scale=float(n) if method == "ward" else 1.0)
+    Z = np.empty((max(n - 1, 0), 4), dtype=np.float64)
+    if n < 2:
+        return Z
+    status = _lib.lib.cuhc_linkage(_lib.float_ptr(points), n, d, METRIC_IDS[metric], float(p),
+                                   LINKAGE_IDS[method], ALGORITHM_IDS[chosen], _lib.double_ptr(Z))
+    if chosen == "dense" and status == _lib.HIP_ERROR_OUT_OF_MEMORY:
+        raise MemoryError(_dense_memory_message(n, method))
     _lib.check(status)
-    return tree
+    return Z
+
+
+def _dense_memory_message(n, method):
+    gigabytes = 2 * n * (n - 1) / 1e9  # the condensed float32 matrix
+    message = (f"cuhc.linkage: the dense distance matrix for {n} points needs {gigabytes:.1f} GB "
+               "of GPU memory, which isn't available")
+    if method == "ward":
+        message += " (ward linkage can use algorithm='rnn', which doesn't need it)"
+    return message
 
 
 def _ward_rnn_with_options(X, min_progress, dense_below=0, allow_dense=True):
     """Test hook: Ward via rnn rounds with explicit hand-over settings."""
-    X = as_points(X, "X")
-    n = X.shape[0]
-    tree = np.empty((max(n - 1, 0), 4), dtype=np.float64)
+    points = as_points(X, "X")
+    n, d = points.shape
+    Z = np.empty((max(n - 1, 0), 4), dtype=np.float64)
     _lib.check(_lib.lib.cuhc_ward_rnn_with_options(
-        _lib.float_ptr(X), n, X.shape[1], float(min_progress), int(dense_below),
-        int(bool(allow_dense)), _lib.double_ptr(tree)))
-    return tree
+        _lib.float_ptr(points), n, d, float(min_progress), int(dense_below),
+        int(bool(allow_dense)), _lib.double_ptr(Z)))
+    return Z
 
 
 def cut_tree(Z, num_merges):
     """Labels 0..k-1 of the leaves after the first num_merges merges of Z.
 
-    Labels follow the order of each flat cluster's root id (np.unique order).
+    Labels follow the order of each flat cluster's top node id (the largest id
+    in it, since a merge's id exceeds its children's).
     """
     Z = np.asarray(Z)
-    n = Z.shape[0] + 1
-    up = np.arange(2 * n - 1)
-    kids = Z[:num_merges, :2].astype(np.intp)
-    new_ids = n + np.arange(num_merges)
-    up[kids[:, 0]] = new_ids
-    up[kids[:, 1]] = new_ids
-    # Pointer doubling until every node points at its flat cluster's root.
-    while True:
-        nxt = up[up]
-        if np.array_equal(nxt, up):
-            break
-        up = nxt
-    return np.unique(up[:n], return_inverse=True)[1]
+    n = len(Z) + 1
+    nodes = 2 * n - 1
+    # The first num_merges merges as edges child -> new cluster; the flat
+    # clusters are the connected components of that forest.
+    made = n + np.arange(num_merges)
+    kids = Z[:num_merges, :2].astype(np.int64)
+    edges = coo_matrix((np.ones(2 * num_merges, dtype=np.int8),
+                        (kids.T.ravel(), np.concatenate([made, made]))), shape=(nodes, nodes))
+    _, component = connected_components(edges, directed=False)
+    top = np.zeros(component.max() + 1, dtype=np.int64)
+    np.maximum.at(top, component, np.arange(nodes))
+    return np.unique(top[component[:n]], return_inverse=True)[1]
 
 
 def fcluster(Z, t, criterion="maxclust"):
@@ -121,13 +129,13 @@ def fcluster(Z, t, criterion="maxclust"):
     array([1, 1, 2, 2])
     """
     Z = np.asarray(Z)
-    n = Z.shape[0] + 1
-    if criterion == "maxclust":
+    n = len(Z) + 1
+    if criterion == "distance":
+        merges = int((Z[:, 2] <= t).sum())
+    elif criterion == "maxclust":
         if t < 1:
             raise ValueError(f"maxclust needs t >= 1, got {t}")
-        num_merges = n - min(int(t), n)
-    elif criterion == "distance":
-        num_merges = int(np.count_nonzero(Z[:, 2] <= t))
+        merges = n - min(int(t), n)
     else:
         raise ValueError(f"unknown criterion {criterion!r}; expected 'maxclust' or 'distance'")
-    return cut_tree(Z, num_merges) + 1
+    return cut_tree(Z, merges) + 1

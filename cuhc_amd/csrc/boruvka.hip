@@ -16,20 +16,29 @@
 
 namespace cuhc {
 
+// Both entry paths check the coordinates first (kErrNonFinite for NaN or
+// infinity): the FP16 candidate kernels are compiled without NaN semantics and
+// the host k-d order needs ordered comparisons.
 int DevicePoints::upload(const float *hostPoints, int n, int d, bool spatial) {
   const size_t count = (size_t)n * d;
   Status s;
   s(points.alloc(count));
   if (!s.ok())
     return s.code;
-  if (!spatial)
-    return cuhc::upload(points, hostPoints, count * sizeof(float));
-  order = spatialOrder(hostPoints, n, d);
-  host = hostPoints;
+  if (!spatial) {
+    s(cuhc::upload(points, hostPoints, count * sizeof(float)));
+    s(checkFinite(points, count));
+    return s.code;
+  }
   DevArray<float> raw;
   s(raw.alloc(count));
   s(orig.alloc(n));
   s(cuhc::upload(raw, hostPoints, count * sizeof(float)));
+  s(checkFinite(raw, count));
+  if (!s.ok())
+    return s.code;
+  order = spatialOrder(hostPoints, n, d);
+  host = hostPoints;
   s(cuhc::upload(orig, order.data(), n * sizeof(int)));
   if (s.ok())
     s(gatherRows(raw, orig, n, d, points));
@@ -37,14 +46,20 @@ int DevicePoints::upload(const float *hostPoints, int n, int d, bool spatial) {
   return s.code;
 }
 
+// The caller may have written devicePoints on another stream of this HIP
+// runtime: the device is synchronised before the library stream reads them.
 int DevicePoints::adopt(const float *devicePoints, int n, int d, bool spatial) {
+  const size_t count = (size_t)n * d;
+  Status s;
+  s((int)hipDeviceSynchronize());
+  s(checkFinite(devicePoints, count));
+  if (!s.ok())
+    return s.code;
   if (!spatial) {
     external = devicePoints;
     return 0;
   }
-  const size_t count = (size_t)n * d;
   hostCopy.resize(count);
-  Status s;
   s(download(hostCopy.data(), devicePoints, count * sizeof(float)));
   if (!s.ok())
     return s.code;

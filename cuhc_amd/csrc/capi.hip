@@ -205,6 +205,48 @@ long long cuhc_condense_tree(const double *tree, long long n, long long minClust
   return (long long)rows.size();
 }
 
+int cuhc_cluster_stability(const long long *parent, const long long *child, const double *lambda,
+                           const long long *size, long long rows, double *stability) {
+  if (rows < 0)
+    return kErrInvalidShape;
+  const std::vector<double> st = clusterStability(CondensedView(parent, child, lambda, size, rows));
+  std::copy(st.begin(), st.end(), stability);
+  return 0;
+}
+
+int cuhc_select_clusters(const long long *parent, const long long *child, const double *lambda,
+                         const long long *size, long long rows, double *stability, int method,
+                         int allowSingleCluster, long long maxClusterSize,
+                         unsigned char *selected) {
+  if (rows < 0)
+    return kErrInvalidShape;
+  if (method != 0 && method != 1)
+    return kErrInvalidAlgorithm;
+  return selectClusters(CondensedView(parent, child, lambda, size, rows), stability, method,
+                        allowSingleCluster != 0, maxClusterSize, selected);
+}
+
+int cuhc_label_points(const long long *parent, const long long *child, const double *lambda,
+                      const long long *size, long long rows, const unsigned char *selected,
+                      int allowSingleCluster, double epsilon, long long *labels) {
+  if (rows < 0)
+    return kErrInvalidShape;
+  labelPoints(CondensedView(parent, child, lambda, size, rows), selected, allowSingleCluster != 0,
+              epsilon, labels);
+  return 0;
+}
+
+int cuhc_membership_probabilities(const long long *parent, const long long *child,
+                                  const double *lambda, const long long *size, long long rows,
+                                  const unsigned char *selected, const long long *labels,
+                                  double *probabilities) {
+  if (rows < 0)
+    return kErrInvalidShape;
+  membershipProbabilities(CondensedView(parent, child, lambda, size, rows), selected, labels,
+                          probabilities);
+  return 0;
+}
+
 int cuhc_set_pruning(int mode) {
   if (mode < 0 || mode > 2)
     return kErrInvalidAlgorithm;
@@ -241,8 +283,17 @@ int cuhc_set_precision(int mode) {
 
 int cuhc_device_alloc(unsigned long long bytes, void **ptr) {
   *ptr = nullptr;
-  return (int)hipMalloc(ptr, bytes);
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e == hipErrorOutOfMemory) {
+    // The library's cache of freed blocks may hold the memory: release it and retry.
+    (void)hipGetLastError();
+    releaseCachedBlocks();
+    e = hipMalloc(ptr, bytes);
+  }
+  return (int)e;
 }
+
+void cuhc_release(void) { releaseLibraryState(); }
 
 int cuhc_device_free(void *ptr) { return ptr ? (int)hipFree(ptr) : 0; }
 

@@ -1,12 +1,12 @@
-// Host C++ tree assembly shared by every linkage path.
-//
-// Edge order, union-find and the scipy-format tree follow reference
-// src/mst.cuh:36-78 and 188-235; orderChainMerges follows src/nn_chain.cuh:178-188;
-// condenseTree follows src/hdbscan.cuh:393-483 (sklearn's _condense_tree).
+// Host assembly of scipy-format merge trees from merges (edges between
+// points), shared by every linkage path. The tree format and the global edge
+// order follow reference src/mst.cuh:36-48 and 204-235.
 #pragma once
 
 #include <stdint.h>
 #include <vector>
+
+#include "extract.hpp"
 
 namespace cuhc {
 
@@ -24,26 +24,13 @@ inline bool edgeLess(const Edge &x, const Edge &y) {
   return x.v < y.v;
 }
 
-// Disjoint sets with path halving and union by size.
-class UnionFind {
-public:
-  explicit UnionFind(int n);
-  int find(int x) {
-    while (parent_[x] != x) {
-      parent_[x] = parent_[parent_[x]];
-      x = parent_[x];
-    }
-    return x;
-  }
-  // Joins the sets of x and y; the new root, or -1 if already joined.
-  int unite(int x, int y);
-
-private:
-  std::vector<int> parent_, size_;
-};
-
-// n - 1 merges with finite heights, each joining two clusters not yet joined.
-bool mergesFormTree(const std::vector<Edge> &merges, int n);
+// Merges that arrive in chain or round order (each merge after those that
+// built its two clusters, heights possibly not monotone) into tree order:
+// checks that there are n - 1 of them with finite heights, each joining two
+// clusters not yet joined; raises every height to at least its children's;
+// stable-sorts by height. Returns false (merges unspecified) when the check
+// fails.
+bool mergesToTreeOrder(std::vector<Edge> &merges, int n);
 
 // Scipy linkage rows {min id, max id, height, size} from merges already in
 // tree order; the cluster made by merge s is n + s.
@@ -51,17 +38,5 @@ void treeFromOrderedMerges(const std::vector<Edge> &merges, int n, double *tree)
 
 // Sorts by edgeLess on the host.
 void sortEdges(std::vector<Edge> &edges);
-
-// Makes heights non-decreasing up the tree (a parent at least its children),
-// then stable-sorts by height: merges in chain/round order -> tree order.
-void orderChainMerges(std::vector<Edge> &merges, int n);
-
-struct CondensedRow {
-  long long parent, child;
-  double lambda;
-  long long size;
-};
-
-std::vector<CondensedRow> condenseTree(const double *tree, long long n, long long minClusterSize);
 
 } // namespace cuhc

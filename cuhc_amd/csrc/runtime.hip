@@ -1,14 +1,19 @@
 #include "runtime.hpp"
 
+#include "common.hpp"
+
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <mutex>
 #include <unordered_map>
 
 namespace cuhc {
 
 namespace {
-hipStream_t gStream = nullptr;
-std::once_flag gStreamOnce;
+std::atomic<hipStream_t> gStream{nullptr};
+std::mutex gStreamMutex;
+bool gExitHandler = false;
 int gNumCUs = 0;
 
 struct PendingEvent {
@@ -24,11 +29,23 @@ std::mutex gTimingMutex;
 } // namespace
 
 hipStream_t stream() {
-  std::call_once(gStreamOnce, [] {
-    if (hipStreamCreateWithFlags(&gStream, hipStreamNonBlocking) != hipSuccess)
-      gStream = nullptr;
-  });
-  return gStream;
+  hipStream_t s = gStream.load(std::memory_order_acquire);
+  if (s)
+    return s;
+  std::lock_guard<std::mutex> lock(gStreamMutex);
+  s = gStream.load(std::memory_order_relaxed);
+  if (!s) {
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+      return nullptr;
+    gStream.store(s, std::memory_order_release);
+    // The HIP runtime is initialised by now, so this handler runs before its
+    // teardown (exit handlers run in reverse order of registration).
+    if (!gExitHandler) {
+      gExitHandler = true;
+      std::atexit(releaseLibraryState);
+    }
+  }
+  return s;
 }
 
 int numCUs() {
@@ -50,7 +67,8 @@ int numCUs() {
 // runs on the one library stream, so a reused block is only touched by work
 // ordered after every earlier use of it. The cache holds at most a quarter of
 // device memory; when hipMalloc fails, the cache is released and the request
-// retried. It is never freed at exit (the HIP runtime may be gone by then).
+// retried. releaseLibraryState() frees it (at exit, before the runtime's own
+// teardown).
 namespace {
 struct DeviceCache {
   std::mutex mutex;
@@ -70,7 +88,8 @@ size_t sizeClass(size_t bytes) {
 }
 
 void releaseCache(DeviceCache &cache) {
-  (void)hipStreamSynchronize(stream());
+  if (hipStream_t st = gStream.load())
+    (void)hipStreamSynchronize(st);
   for (auto &entry : cache.blocks)
     for (void *p : entry.second)
       (void)hipFree(p);
@@ -147,7 +166,7 @@ struct PinnedStage {
 };
 
 PinnedStage &pinnedStage() {
-  static PinnedStage *stage = new PinnedStage; // kept until the process ends
+  static PinnedStage *stage = new PinnedStage; // the buffer is freed by releaseLibraryState
   return *stage;
 }
 
@@ -185,6 +204,65 @@ int downloadStaged(const void *src, size_t bytes,
   if (s == 0)
     consume(st.p);
   return s;
+}
+
+namespace {
+
+// flag = 1 when some value has all exponent bits set (infinity or NaN). The
+// test is on the bits, so it holds under any floating-point compile flags.
+__global__ void __launch_bounds__(256)
+    nonFiniteKernel(const unsigned int *__restrict__ bits, size_t count, int *flag) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  bool bad = false;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
+    bad |= (bits[i] & 0x7f800000u) == 0x7f800000u;
+  if (bad)
+    *flag = 1;
+}
+
+} // namespace
+
+int checkFinite(const float *X, size_t count) {
+  if (count == 0)
+    return 0;
+  DevArray<int> flag;
+  Status s;
+  s(flag.alloc(1));
+  s(fill(flag, 0, sizeof(int)));
+  if (!s.ok())
+    return s.code;
+  const size_t blocks = std::min<size_t>((count + 255) / 256, (size_t)numCUs() * 16);
+  hipLaunchKernelGGL(nonFiniteKernel, dim3((unsigned)blocks), dim3(256), 0, stream(),
+                     reinterpret_cast<const unsigned int *>(X), count, flag.get());
+  s((int)hipGetLastError());
+  int bad = 0;
+  s(download(&bad, flag, sizeof(int)));
+  if (!s.ok())
+    return s.code;
+  return bad ? kErrNonFinite : 0;
+}
+
+void releaseCachedBlocks() {
+  DeviceCache &cache = deviceCache();
+  std::lock_guard<std::mutex> lock(cache.mutex);
+  releaseCache(cache);
+}
+
+void releaseLibraryState() {
+  releaseCachedBlocks();
+  {
+    PinnedStage &st = pinnedStage();
+    std::lock_guard<std::mutex> lock(st.mutex);
+    if (st.p)
+      (void)hipHostFree(st.p);
+    st.p = nullptr;
+    st.bytes = 0;
+  }
+  std::lock_guard<std::mutex> lock(gStreamMutex);
+  if (hipStream_t s = gStream.exchange(nullptr)) {
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+  }
 }
 
 int fill(void *dst, int byte, size_t bytes) {

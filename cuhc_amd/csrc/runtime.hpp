@@ -75,6 +75,18 @@ int downloadStaged(const void *src, size_t bytes, const std::function<void(const
 int fill(void *dst, int byte, size_t bytes);
 int sync();
 
+// 0 when all count floats at X (device) are finite, kErrNonFinite (-6) when
+// one is NaN or infinite, or a HIP error. One read of the array.
+int checkFinite(const float *X, size_t count);
+
+// Releases what the library holds between calls: the cached device blocks,
+// the pinned staging buffer and the library stream (recreated on next use).
+// Runs from an exit handler registered when the stream is first created, so
+// that it precedes the HIP runtime's own teardown.
+void releaseLibraryState();
+// Frees the cached device blocks only.
+void releaseCachedBlocks();
+
 // Per-phase GPU timing with HIP events on the library stream. Off by default;
 // cuhc_set_timing(1) turns it on. A phase is bracketed by begin()/end(); the
 // elapsed time is collected at the next collect() (after a stream sync).

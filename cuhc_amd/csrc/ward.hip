@@ -513,12 +513,10 @@ int nnChainWard(const float *X, int n, int d, double *tree) {
   std::vector<Edge> merges;
   if (s.ok())
     s(nnChainMerges(c, nullptr, n, d, merges));
-  if (s.ok() && !mergesFormTree(merges, n))
+  if (s.ok() && !mergesToTreeOrder(merges, n))
     s(kErrNonFinite);
-  if (s.ok()) {
-    orderChainMerges(merges, n);
+  if (s.ok())
     treeFromOrderedMerges(merges, n, tree);
-  }
   return s.code;
 }
 
@@ -613,12 +611,10 @@ int rnnWard(const float *X, int n, int d, const RnnOptions &opt, double *tree) {
     s(denseTail(cur, m, d, n, all));
   else if (tail == kChain)
     s(chainTail(cur, m, d, all));
-  if (s.ok() && !mergesFormTree(all, n))
+  if (s.ok() && !mergesToTreeOrder(all, n))
     s(kErrNonFinite);
-  if (s.ok()) {
-    orderChainMerges(all, n);
+  if (s.ok())
     treeFromOrderedMerges(all, n, tree);
-  }
   return s.code;
 }
 

@@ -10,6 +10,9 @@
  * memory); -1 invalid metric, -2 invalid shape, -3 invalid linkage,
  * -4 invalid algorithm (or core method) for the request, -5 disconnected (some
  * points have no finite distance to the rest), -6 non-finite merge heights.
+ * cuhc_linkage, cuhc_ward_rnn_with_options, cuhc_hdbscan_tree and their
+ * *_device forms return -6 before any distance is computed when X holds NaN or
+ * infinity (the reference leaves that check to its Python layer).
  *
  * ID tables. metric: 0 euclidean, 1 sqeuclidean, 2 manhattan, 3 chebyshev,
  * 4 minkowski (metricParam = p), 5 cosine. linkage: 0 single, 1 complete,
@@ -65,15 +68,43 @@ const char *cuhc_error_string(int code);
 
 /* ---- Extensions (no counterpart in the reference) ---------------------- */
 
+/* HDBSCAN cluster extraction on the host, the steps after cuhc_condense_tree
+ * (the reference runs them in NumPy, python/cuhc/_hdbscan.py:148-350, with
+ * the results of scikit-learn's _tree.pyx). The condensed tree is given as its
+ * four columns of `rows` rows; root = min(parent), top = max(parent).
+ * cuhc_cluster_stability: stability[c - root] for clusters root .. top.
+ * cuhc_select_clusters: method 0 excess of mass, 1 leaf, without the epsilon
+ * rule (the caller applies it); stability (as above) is updated as excess of
+ * mass does; selected[c - root] = 1 for chosen clusters; maxClusterSize <= 0
+ * means no limit. cuhc_label_points: labels[p] for points 0 .. root - 1 (-1 =
+ * noise, else the rank of the point's cluster among the selected ids).
+ * cuhc_membership_probabilities: probabilities[p] for those labels. Return 0
+ * or a status. */
+int cuhc_cluster_stability(const long long *parent, const long long *child, const double *lambda,
+                           const long long *size, long long rows, double *stability);
+int cuhc_select_clusters(const long long *parent, const long long *child, const double *lambda,
+                         const long long *size, long long rows, double *stability, int method,
+                         int allowSingleCluster, long long maxClusterSize,
+                         unsigned char *selected);
+int cuhc_label_points(const long long *parent, const long long *child, const double *lambda,
+                      const long long *size, long long rows, const unsigned char *selected,
+                      int allowSingleCluster, double epsilon, long long *labels);
+int cuhc_membership_probabilities(const long long *parent, const long long *child,
+                                  const double *lambda, const long long *size, long long rows,
+                                  const unsigned char *selected, const long long *labels,
+                                  double *probabilities);
+
 /* Precision of the single-linkage Borůvka passes: 0 auto (FP16 MFMA
  * candidates + certified FP32 recheck for euclidean/sqeuclidean, d >= 16),
  * 1 plain FP32 tiles, 2 mixed whenever the metric allows. The tree is
  * identical in every mode. For benchmarks and tests. */
 int cuhc_set_precision(int mode);
 
-/* cuhc_linkage with X already in device memory (n x d floats, row-major, from
- * hipMalloc or torch). tree is host memory. For callers that keep points
- * resident in HBM (zero-copy input). */
+/* cuhc_linkage with X already in device memory (n x d floats, row-major,
+ * allocated in this process's HIP runtime, e.g. by cuhc_device_alloc). tree is
+ * host memory. The call synchronises the device before reading X, so X may
+ * have been written on any stream. For callers that keep points resident in
+ * HBM (zero-copy input). */
 int cuhc_linkage_device(const float *deviceX, int n, int d, int metricId, float metricParam,
                         int linkageId, int algorithmId, double *tree);
 
@@ -83,7 +114,9 @@ int cuhc_hdbscan_tree_device(const float *deviceX, int n, int d, int metricId, f
                              float *core);
 
 /* Device buffers on the library's HIP runtime, for callers that keep inputs
- * resident in HBM across calls (the *_device entry points). Return a status. */
+ * resident in HBM across calls (the *_device entry points). Return a status.
+ * cuhc_device_alloc releases the library's cache of freed blocks and retries
+ * when device memory is short. */
 int cuhc_device_alloc(unsigned long long bytes, void **ptr);
 int cuhc_device_free(void *ptr);
 int cuhc_copy_to_device(void *dst, const void *src, unsigned long long bytes);
@@ -91,6 +124,11 @@ int cuhc_copy_to_host(void *dst, const void *src, unsigned long long bytes);
 /* Device-to-device copy (a plain copy kernel of known size: the benchmark
  * uses it to scale the HBM byte counters). No counterpart in the reference. */
 int cuhc_copy_device(void *dst, const void *src, unsigned long long bytes);
+
+/* Frees what the library keeps between calls: its cache of freed device
+ * blocks, the pinned staging buffer and its HIP stream (recreated on the next
+ * call). Also runs from an exit handler, before the HIP runtime's teardown. */
+void cuhc_release(void);
 
 /* Per-phase GPU timing with HIP events on the library stream (0 = off).
  * Phases: 0 FP32 distance tiles, 1 FP16 MFMA candidates, 2 FP32 recheck,

@@ -76,8 +76,8 @@ def test_argument_codes_before_any_gpu_work():
 def test_condense_matches_sklearn_golden(mcs):
     ref = GOLDEN[f"condensed_{mcs}"]
     for parents, children, lambdas, sizes in [_hdbscan.condense_tree(GOLDEN["extract_Z"], mcs),
-                                              _hdbscan.condense_tree_numpy(GOLDEN["extract_Z"],
-                                                                           mcs)]:
+                                              _hdbscan.condense_tree_reference(
+                                                  GOLDEN["extract_Z"], mcs)]:
         np.testing.assert_array_equal(parents, ref[:, 0])
         np.testing.assert_array_equal(children, ref[:, 1])
         np.testing.assert_array_equal(lambdas, ref[:, 2])
@@ -202,3 +202,58 @@ def test_estimators_are_sklearn_estimators():
         assert clone(ours).get_params() == ours.get_params()
         for name in names:
             assert getattr(ours, name) == getattr(ref, name), name
+
+
+def _sklearn_tree_to_labels(Z, **params):
+    from sklearn.cluster._hdbscan._linkage import HIERARCHY_dtype
+    from sklearn.cluster._hdbscan._tree import tree_to_labels as sk_tree_to_labels
+    hierarchy = np.empty(len(Z), dtype=HIERARCHY_dtype)
+    hierarchy["left_node"] = Z[:, 0]
+    hierarchy["right_node"] = Z[:, 1]
+    hierarchy["value"] = Z[:, 2]
+    hierarchy["cluster_size"] = Z[:, 3]
+    import warnings
+    with warnings.catch_warnings():  # scikit-learn's own epsilon search converts 1-element arrays
+        warnings.simplefilter("ignore", DeprecationWarning)
+        return sk_tree_to_labels(hierarchy, **params)
+
+
+def _blob_tree(seed, n):
+    # Single-linkage trees of blobs plus noise with ties (rounded coordinates).
+    from scipy.cluster.hierarchy import linkage as scipy_linkage
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(-8, 8, (rng.integers(2, 7), 2))
+    X = np.vstack([c + rng.normal(0, rng.uniform(0.2, 1.2), (n // len(centres), 2))
+                   for c in centres] + [rng.uniform(-10, 10, (n // 10, 2))])
+    return scipy_linkage(np.round(X, 1), "single")
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_extraction_matches_sklearn_on_random_trees(seed):
+    # The C++ extraction against scikit-learn's own tree_to_labels, every
+    # selection option, on trees with tied heights.
+    Z = _blob_tree(seed, 400 + 150 * seed)
+    options = [dict(min_cluster_size=m, cluster_selection_method=meth, allow_single_cluster=a,
+                    cluster_selection_epsilon=e, max_cluster_size=top)
+               for m in (3, 12) for meth in ("eom", "leaf") for a in (False, True)
+               for e in (0.0, 0.35) for top in (None, 80)]
+    for params in options:
+        ours_l, ours_p = _hdbscan.tree_to_labels(Z, **params)
+        ref_l, ref_p = _sklearn_tree_to_labels(Z, **params)
+        np.testing.assert_array_equal(ours_l, ref_l, err_msg=str(params))
+        np.testing.assert_allclose(ours_p, ref_p, rtol=0, atol=1e-12, err_msg=str(params))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_condensed_tree_two_methods_agree_with_sklearn(seed):
+    from sklearn.cluster._hdbscan._linkage import HIERARCHY_dtype
+    from sklearn.cluster._hdbscan._tree import _condense_tree
+    Z = _blob_tree(10 + seed, 700)
+    hierarchy = np.empty(len(Z), dtype=HIERARCHY_dtype)
+    for i, name in enumerate(["left_node", "right_node", "value", "cluster_size"]):
+        hierarchy[name] = Z[:, i]
+    for mcs in (2, 5, 40):
+        ref = _condense_tree(hierarchy, mcs)
+        for ours in (_hdbscan.condense_tree(Z, mcs), _hdbscan.condense_tree_reference(Z, mcs)):
+            for col, name in zip(ours, ["parent", "child", "value", "cluster_size"]):
+                np.testing.assert_array_equal(col, ref[name])
