@@ -10,7 +10,12 @@ timed once beside it and the trees must be identical.
 
 Prints one JSON line (rank 0). --gpus > 1 is not supported (one GPU).
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--plain-steps P]
+The same line carries the host-array time of the drop-in entry point
+(cuhc_linkage, PCIe upload included), the 1M-point 8-d HDBSCAN trees per
+core-distance method (auto, learned radius, radix select) with their identity,
+and the CPU baseline (scikit-learn's Prim MST, one thread, on a prefix).
+
+    python bench.py [--steps K] [--warmup W] [--n N] [--plain-steps P] [--no-c3]
 """
 
 import argparse
@@ -85,17 +90,83 @@ def timed_steps(fn, steps):
     return (time.perf_counter() - t0) / steps
 
 
+def host_info():
+    """Cores, CPU model and memory of the machine the CPU baseline runs on."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["cpu_model"] = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        for line in pathlib.Path("/proc/meminfo").read_text().splitlines():
+            if line.startswith("MemTotal:"):
+                info["ram_gib"] = round(int(line.split()[1]) / 2**20, 1)
+    except OSError:
+        pass
+    return info
+
+
+def sklearn_single_linkage(X64):
+    """scikit-learn's CPU single-linkage tree: Prim's MST over the data matrix
+    (O(n) memory), edges stably sorted by weight, then make_single_linkage."""
+    from sklearn.cluster._hdbscan._linkage import make_single_linkage, mst_from_data_matrix
+    from sklearn.metrics import DistanceMetric
+    mst = mst_from_data_matrix(X64, np.zeros(len(X64)), DistanceMetric.get_metric("euclidean"))
+    mst = mst[np.argsort(mst["distance"], kind="mergesort")]
+    h = make_single_linkage(mst)
+    return np.column_stack([h["left_node"], h["right_node"], h["value"], h["cluster_size"]])
+
+
 def cpu_baseline(d, seed, sample_n, full_n):
-    """The CPU oracle (Prim's MST in FP32, one thread) on a bounded prefix of the
-    same data; seconds per tree extrapolated to full_n by n^2 (its cost)."""
-    import oracle
+    """The reference's CPU comparison path for this workload (scikit-learn's
+    Prim MST + single-linkage tree, one thread: BLAS pinned by threadpoolctl)
+    on a bounded prefix of the same data, extrapolated to full_n by n^2 (the
+    cost of Prim's algorithm on a dense graph). The prefix's GPU tree is
+    compared with scikit-learn's FP64 tree."""
+    from threadpoolctl import threadpool_limits
+
+    import cuhc_amd as cuhc
     X = make_points(full_n, d, seed)[:sample_n].copy()
-    t0 = time.perf_counter()
-    oracle.single_linkage(X)
-    t = time.perf_counter() - t0
-    return {"value": t * (full_n / sample_n) ** 2, "unit": "s", "cores": 1, "kind": "port",
-            "sample": f"oracle Prim single linkage on the first {sample_n} points "
-                      f"(d={d}) in {t:.2f} s, extrapolated to n={full_n} by n^2"}
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        Zs = sklearn_single_linkage(X.astype(np.float64))
+        t = time.perf_counter() - t0
+    Zg = cuhc.linkage(X, method="single")
+    ours, theirs = tree_clusters(Zg), tree_clusters(Zs)
+    common = sorted(ours.keys() & theirs.keys())
+    a = np.array([ours[c] for c in common])
+    b = np.array([theirs[c] for c in common])
+    out = {"value": t * (full_n / sample_n) ** 2, "unit": "s", "cores": 1, "kind": "reference",
+           "sample": f"scikit-learn {_sklearn_version()} mst_from_data_matrix + "
+                     f"make_single_linkage (FP64, threadpool_limits(1)) on the first "
+                     f"{sample_n} points (d={d}) in {t:.2f} s, extrapolated to n={full_n} by n^2",
+           "host": host_info(),
+           "prefix_tree_vs_sklearn": {
+               "clusters_differing_by_tie_order": len(ours.keys() - theirs.keys()),
+               "max_rel_height_error": float((np.abs(a - b) / b).max())}}
+    return out
+
+
+def tree_clusters(Z):
+    """{(smallest point, size): height} for every cluster a merge tree creates
+    (tied merges may come in either order; the clusters and heights agree)."""
+    n = len(Z) + 1
+    smallest = np.empty(2 * n - 1, dtype=np.int64)
+    smallest[:n] = np.arange(n)
+    out = {}
+    for i, (a, b, h, size) in enumerate(np.asarray(Z).tolist()):
+        smallest[n + i] = min(smallest[int(a)], smallest[int(b)])
+        out[(int(smallest[n + i]), int(size))] = h
+    return out
+
+
+def _sklearn_version():
+    import sklearn
+    return sklearn.__version__
 
 
 def make_c3(seed=2):
@@ -109,49 +180,69 @@ def make_c3(seed=2):
     return np.vstack([X, noise]).astype(np.float32)
 
 
-def other_config(name, args):
-    """Seconds per call for the other configurations (parity-test cases of the
-    same path; printed as one JSON line, not the headline)."""
+C3_WORKLOAD = "HDBSCAN mcs=min_samples=25, n=1,000,000, d=8, 20 blobs + 10% noise"
+CORE_METHODS = {"auto": 0, "fused": 1, "radix": 2, "learned": 3}
+
+
+def c3_block(methods, steps, radix_steps=1):
+    """HDBSCAN trees of the 1M-point 8-d workload from points resident in HBM,
+    one entry per core-distance method (auto: fused k-d top-k + pruned
+    Boruvka; learned: the radius network's proposals, verified; radix: the
+    full-row radix select), with the identity of their trees and core
+    distances and the extraction time."""
     from cuhc_amd import _lib
     from cuhc_amd._hdbscan import tree_to_labels
     lib = _lib.lib
-    out = {"config": name}
-    if name == "c3":
-        X = make_c3()
-        n, d, k = len(X), 8, 25
-        dev = _lib.DeviceArray(X)
-        tree = np.empty((n - 1, 4))
-        core = np.empty(n, np.float32)
-        methods = {"auto": 0, "fused": 1, "radix": 2, "learned": 3}
-        trees = {}
-        for method in args.core_methods.split(","):
+    X = make_c3()
+    n, d, k = len(X), 8, 25
+    dev = _lib.DeviceArray(X)
+    tree = np.empty((n - 1, 4))
+    core = np.empty(n, np.float32)
+    out, trees = {"workload": C3_WORKLOAD}, {}
+    try:
+        for method in methods:
             call = lambda: _lib.check(lib.cuhc_hdbscan_tree_device(  # noqa: E731
-                dev.ptr, n, d, 0, 2.0, k, 1.0, methods[method], _lib.double_ptr(tree),
+                dev.ptr, n, d, 0, 2.0, k, 1.0, CORE_METHODS[method], _lib.double_ptr(tree),
                 _lib.float_ptr(core)))
-            call()
+            reps = radix_steps if method == "radix" else steps
+            if method != "radix":
+                call()
             _lib.set_timing(True)
             _lib.reset_timings()
             _lib.prune_stats()
             _lib.learned_stats()
-            t = timed_steps(call, args.steps)
-            out[method] = {"tree_s": t, "phases_ms": {
-                k2: round(v[0] / args.steps, 2) for k2, v in _lib.read_timings().items() if v[1]},
+            t = timed_steps(call, reps)
+            out[method] = {"tree_s": t, "steps": reps, "phases_ms": {
+                k2: round(v[0] / reps, 2) for k2, v in _lib.read_timings().items() if v[1]},
                 "prune_stats": _lib.prune_stats()}
             if method == "learned":
                 out[method]["learned_stats_per_call"] = {
-                    k2: v / args.steps for k2, v in _lib.learned_stats().items()}
+                    k2: v / reps for k2, v in _lib.learned_stats().items()}
             _lib.set_timing(False)
             trees[method] = (tree.copy(), core.copy())
             log(f"c3 {method}: {out[method]}")
-        first = next(iter(trees.values()))
-        out["identical_across_methods"] = all(
-            np.array_equal(a, first[0]) and np.array_equal(c, first[1])
-            for a, c in trees.values())
-        t0 = time.perf_counter()
-        labels, _ = tree_to_labels(tree, 25)
-        out.update(extraction_s=time.perf_counter() - t0,
-                   clusters=int(labels.max() + 1), noise=int((labels < 0).sum()),
-                   workload="HDBSCAN mcs=min_samples=25, n=1,000,000, d=8, 20 blobs + 10% noise")
+    finally:
+        dev.free()
+    first = next(iter(trees.values()))
+    out["identical_across_methods"] = all(
+        np.array_equal(a, first[0]) and np.array_equal(c, first[1]) for a, c in trees.values())
+    if "auto" in out and "learned" in out:
+        out["learned_vs_auto"] = out["learned"]["tree_s"] / out["auto"]["tree_s"]
+    t0 = time.perf_counter()
+    labels, _ = tree_to_labels(first[0], 25)
+    out.update(extraction_s=time.perf_counter() - t0,
+               clusters=int(labels.max() + 1), noise=int((labels < 0).sum()))
+    return out
+
+
+def other_config(name, args):
+    """Seconds per call for the other configurations (parity-test cases of the
+    same path; printed as one JSON line, not the headline)."""
+    from cuhc_amd import _lib
+    lib = _lib.lib
+    out = {"config": name}
+    if name == "c3":
+        out.update(c3_block(args.core_methods.split(","), args.steps))
     else:
         spec = {"c1": (20_000, 32, 0, 0, 1, 0, "single/mst, n=20k, d=32"),
                 "c2": (100_000, 32, 1, 3, 3, 0, "ward/rnn, n=100k, d=32"),
@@ -171,6 +262,7 @@ def other_config(name, args):
         out["phases_ms"] = {k2: round(v[0] / args.steps, 2) for k2, v in
                             _lib.read_timings().items() if v[1]}
         _lib.set_timing(False)
+        dev.free()
         out.update(tree_s=t, workload=label)
     print(json.dumps(out), flush=True)
 
@@ -187,8 +279,12 @@ def main():
     parser.add_argument("--plain-steps", type=int, default=1)
     parser.add_argument("--cpu-sample", type=int, default=20_000)
     parser.add_argument("--no-cpu", action="store_true")
-    parser.add_argument("--core-methods", default="auto,learned",
+    parser.add_argument("--core-methods", default="auto,learned,radix",
                         help="c3: comma-separated core-distance methods to time")
+    parser.add_argument("--no-c3", action="store_true",
+                        help="leave out the nested HDBSCAN (1M, d=8) block of the default run")
+    parser.add_argument("--host-steps", type=int, default=2,
+                        help="trees timed from host memory (cuhc_linkage, PCIe included)")
     parser.add_argument("--calibrate-copy-mib", type=int, default=0,
                         help="only run a device-to-device copy of this size (counter scaling)")
     args = parser.parse_args()
@@ -263,7 +359,25 @@ def main():
         result["speedup_vs_plain"] = s_plain / s_mixed
         result["tree_identical_to_plain"] = bool(np.array_equal(tree, z_mixed))
         log(f"plain: {s_plain:.3f} s/tree, identical={result['tree_identical_to_plain']}")
+
+    if args.host_steps > 0:
+        # The drop-in entry point: points in host memory, so every tree adds
+        # the upload of n*d*4 bytes over PCIe. Reported beside value, not as it.
+        _lib.set_precision("mixed")
+        log(f"timing {args.host_steps} steps (mixed, host-array cuhc_linkage)")
+        call = lambda: _lib.check(lib.cuhc_linkage(  # noqa: E731
+            _lib.float_ptr(X), n, d, 0, 2.0, 0, 1, _lib.double_ptr(tree)))
+        s_host = timed_steps(call, args.host_steps)
+        result["host_array"] = {
+            "entry_point": "cuhc_linkage", "s_per_tree": s_host,
+            "upload_bytes": n * d * 4, "tree_identical": bool(np.array_equal(tree, z_mixed))}
+        log(f"host array: {s_host:.3f} s/tree")
     _lib.set_precision("auto")
+    dev.free()
+
+    if not args.no_c3:
+        log("c3 block")
+        result["c3"] = c3_block(args.core_methods.split(","), args.steps)
 
     if not args.no_cpu:
         log("cpu baseline")

@@ -336,13 +336,26 @@ int coreFused(const float *X, int n, int d, MetricSpec m, int k, float *core, bo
   });
 }
 
-int coreRadix(const float *X, int n, int d, MetricSpec m, int k, float *core) {
-  const size_t maxBuffer = (size_t)512 << 20;
-  const int tileRows = tilePoints(m.id);
+// Distance rows per radix-select launch: a quarter of the free device memory
+// (at most 64 GiB; at n = 10^6 that is 16,384 rows of 4 MB, so one launch
+// fills the chip many times over), at least one row, a multiple of the tile
+// rows when possible, never more than wanted.
+int radixChunkRows(int n, int wanted, int tileRows) {
+  size_t freeBytes = 0, totalBytes = 0;
+  if (hipMemGetInfo(&freeBytes, &totalBytes) != hipSuccess) {
+    (void)hipGetLastError();
+    freeBytes = (size_t)4 << 30;
+  }
+  const size_t budget = std::min(freeBytes / 4, (size_t)64 << 30);
   const size_t rowBytes = (size_t)n * sizeof(float);
-  int rows = (int)std::min<size_t>(maxBuffer / rowBytes, (size_t)n);
-  rows = std::max(tileRows, rows / tileRows * tileRows);
-  rows = std::min(rows, n);
+  size_t rows = std::max<size_t>(1, budget / rowBytes);
+  if (rows >= (size_t)tileRows)
+    rows = rows / tileRows * tileRows;
+  return (int)std::min<size_t>(rows, (size_t)wanted);
+}
+
+int coreRadix(const float *X, int n, int d, MetricSpec m, int k, float *core) {
+  const int rows = radixChunkRows(n, n, tilePoints(m.id));
   DevArray<float> chunk;
   Status s;
   s(chunk.alloc((size_t)rows * n));
@@ -407,9 +420,7 @@ int coreForRows(const float *X, int n, int d, MetricSpec m, int k, const int *ro
     return 0;
   if (k <= kMaxFusedK)
     return coreForRowsFused(X, n, d, m, k, rows, numRows, core);
-  const size_t maxBuffer = (size_t)512 << 20;
-  const size_t rowBytes = (size_t)n * sizeof(float);
-  int chunkRows = (int)std::max<size_t>(1, std::min<size_t>(maxBuffer / rowBytes, numRows));
+  const int chunkRows = radixChunkRows(n, numRows, 1);
   DevArray<float> chunk, rowData;
   Status s;
   s(chunk.alloc((size_t)chunkRows * n));
