@@ -265,11 +265,41 @@ void releaseLibraryState() {
   }
 }
 
+namespace {
+
+// Byte fill: 16-byte stores over the aligned middle, single bytes at the ends.
+__global__ void __launch_bounds__(256)
+    fillKernel(unsigned char *__restrict__ dst, size_t bytes, unsigned int byte) {
+  const size_t head = std::min<size_t>(bytes, (16 - ((uintptr_t)dst & 15)) & 15);
+  const size_t words = (bytes - head) / 16;
+  const size_t tail = head + words * 16;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned int v = byte * 0x01010101u;
+  uint4 *body = reinterpret_cast<uint4 *>(dst + head);
+  for (size_t i = tid; i < words; i += stride)
+    body[i] = make_uint4(v, v, v, v);
+  if (tid < head)
+    dst[tid] = (unsigned char)byte;
+  if (tid < bytes - tail)
+    dst[tail + tid] = (unsigned char)byte;
+}
+
+} // namespace
+
+// A kernel rather than hipMemsetAsync: the same stream-ordered semantics, and
+// a plain kernel node wherever a caller captures the library's work.
 int fill(void *dst, int byte, size_t bytes) {
   if (bytes == 0)
     return 0;
-  return (int)hipMemsetAsync(dst, byte, bytes, stream());
+  const size_t words = bytes / 16 + 1;
+  const size_t blocks = std::min<size_t>((words + 255) / 256, (size_t)numCUs() * 16);
+  hipLaunchKernelGGL(fillKernel, dim3((unsigned)std::max<size_t>(blocks, 1)), dim3(256), 0,
+                     stream(), static_cast<unsigned char *>(dst), bytes,
+                     (unsigned int)(byte & 0xff));
+  return (int)hipGetLastError();
 }
+
 
 int sync() { return (int)hipStreamSynchronize(stream()); }
 
