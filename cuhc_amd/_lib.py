@@ -53,6 +53,8 @@ _declare("cuhc_membership_probabilities", _int, *_CONDENSED, _u8p, _i64p, _f64p)
 _declare("cuhc_set_precision", _int, _int)
 _declare("cuhc_linkage_device", _int, ctypes.c_void_p, _int, _int, _int, _float, _int, _int,
          _f64p)
+_declare("cuhc_linkage_device_noalloc", _int, ctypes.c_void_p, _int, _int, _int, _float, _int,
+         _int, _f64p)
 _declare("cuhc_hdbscan_tree_device", _int, ctypes.c_void_p, _int, _int, _int, _float, _int,
          _float, _int, _f64p, _f32p)
 _declare("cuhc_device_alloc", _int, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_void_p))

@@ -38,7 +38,7 @@ DEVICE = [
 
 SOURCES = [
     "runtime.hip", "distance.hip", "boruvka.hip", "core.hip", "dense.hip", "ward.hip",
-    "mfma.hip", "mlp.hip", "learned.hip", "edge_sort.hip", "contract.hip", "capi.hip", "host_tree.cpp", "extract.cpp",
+    "mfma.hip", "mlp.hip", "learned.hip", "edge_sort.hip", "contract.hip", "capi.hip", "noalloc.hip", "host_tree.cpp", "extract.cpp",
 ]
 
 # Per-file extra flags. mfma.hip: inputs are validated finite, so min/max need

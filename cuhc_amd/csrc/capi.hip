@@ -125,6 +125,11 @@ int hdbscanImpl(const float *X, bool onDevice, int n, int d, int metricId, float
 
 } // namespace
 
+int cuhc::linkageFromDevicePoints(const float *X, int n, int d, int metricId, float metricParam,
+                                  int linkageId, int algorithmId, double *tree) {
+  return linkageImpl(X, true, n, d, metricId, metricParam, linkageId, algorithmId, tree);
+}
+
 extern "C" {
 
 int cuhc_pairwise_distances(const float *A, int nA, const float *B, int nB, int d, int metricId,
@@ -270,6 +275,9 @@ const char *cuhc_error_string(int code) {
   case kErrNonFinite: return "distances overflowed float32 or are NaN; rescale the data";
   case kErrDisconnected:
     return "some points have no finite distance to the others (NaN or infinite values?)";
+  case kErrWorkspace:
+    return "input exceeds the fixed workspace of cuhc_linkage_device_noalloc "
+           "(at most 65536 points of at most 64 features)";
   default: return hipGetErrorString((hipError_t)code);
   }
 }

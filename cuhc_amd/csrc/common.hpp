@@ -37,6 +37,7 @@ constexpr int kErrInvalidLinkage = -3;
 constexpr int kErrInvalidAlgorithm = -4;
 constexpr int kErrDisconnected = -5;
 constexpr int kErrNonFinite = -6;
+constexpr int kErrWorkspace = -7; // no counterpart: the fixed workspace is too small
 
 enum MetricId : int {
   kEuclidean = 0,

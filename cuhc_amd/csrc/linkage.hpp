@@ -9,6 +9,11 @@ namespace cuhc {
 enum LinkageId : int { kSingle = 0, kComplete = 1, kAverage = 2, kWard = 3 };
 enum AlgorithmId : int { kAlgDense = 0, kAlgMst = 1, kAlgNnChain = 2, kAlgRnn = 3 };
 
+// cuhc_linkage_device's body (capi.hip): validation, then the algorithm, on
+// points already in device memory.
+int linkageFromDevicePoints(const float *X, int n, int d, int metricId, float metricParam,
+                            int linkageId, int algorithmId, double *tree);
+
 // Full merge tree from the points through the condensed distance matrix
 // (n (n - 1) / 2 floats in HBM) and the Lance-Williams updates. tree: host,
 // (n - 1) x 4. Returns hipErrorOutOfMemory (2) if the matrix does not fit.

@@ -98,7 +98,41 @@ void releaseCache(DeviceCache &cache) {
 }
 } // namespace
 
+namespace {
+struct Workspace {
+  char *base = nullptr;
+  size_t bytes = 0, used = 0;
+};
+thread_local Workspace *tWorkspace = nullptr;
+constexpr size_t kSliceAlign = 256;
+size_t sliceBytes(size_t bytes) { return (bytes + kSliceAlign - 1) / kSliceAlign * kSliceAlign; }
+} // namespace
+
+WorkspaceScope::WorkspaceScope(void *base, size_t bytes) {
+  Workspace *w = new Workspace;
+  w->base = static_cast<char *>(base);
+  w->bytes = bytes;
+  tWorkspace = w;
+}
+
+WorkspaceScope::~WorkspaceScope() {
+  delete tWorkspace;
+  tWorkspace = nullptr;
+}
+
+bool inWorkspace() { return tWorkspace != nullptr; }
+
 int deviceAlloc(void **p, size_t bytes) {
+  if (Workspace *w = tWorkspace) {
+    const size_t want = sliceBytes(bytes);
+    if (want > w->bytes - w->used) {
+      *p = nullptr;
+      return kErrWorkspace;
+    }
+    *p = w->base + w->used;
+    w->used += want;
+    return 0;
+  }
   DeviceCache &cache = deviceCache();
   const size_t cls = sizeClass(bytes);
   std::lock_guard<std::mutex> lock(cache.mutex);
@@ -119,6 +153,14 @@ int deviceAlloc(void **p, size_t bytes) {
 }
 
 void deviceFree(void *p, size_t bytes) {
+  if (Workspace *w = tWorkspace) {
+    char *q = static_cast<char *>(p);
+    if (q >= w->base && q < w->base + w->bytes) {
+      if (q + sliceBytes(bytes) == w->base + w->used)
+        w->used = (size_t)(q - w->base);
+      return;
+    }
+  }
   DeviceCache &cache = deviceCache();
   const size_t cls = sizeClass(bytes);
   std::lock_guard<std::mutex> lock(cache.mutex);
@@ -179,6 +221,13 @@ int downloadStaged(const void *src, size_t bytes,
   if (bytes == 0) {
     consume(nullptr);
     return 0;
+  }
+  if (inWorkspace()) {
+    std::vector<char> host(bytes);
+    const int s = download(host.data(), src, bytes);
+    if (s == 0)
+      consume(host.data());
+    return s;
   }
   PinnedStage &st = pinnedStage();
   std::lock_guard<std::mutex> lock(st.mutex);
@@ -300,7 +349,6 @@ int fill(void *dst, int byte, size_t bytes) {
   return (int)hipGetLastError();
 }
 
-
 int sync() { return (int)hipStreamSynchronize(stream()); }
 
 bool timingEnabled() { return gTiming; }
@@ -312,7 +360,8 @@ void setTiming(bool on) {
 
 thread_local int tDepth[kNumPhases];
 
-PhaseTimer::PhaseTimer(int p) : phase(p), active(gTiming), counted(gTiming) {
+PhaseTimer::PhaseTimer(int p)
+    : phase(p), active(gTiming && !inWorkspace()), counted(gTiming && !inWorkspace()) {
   if (!active)
     return;
   if (tDepth[p]++ > 0) {

@@ -27,6 +27,21 @@ struct Status {
   bool ok() const { return code == 0; }
 };
 
+// Fixed device workspace. While a WorkspaceScope lives on a thread,
+// deviceAlloc on that thread hands out 256-byte aligned slices of
+// [base, base + bytes) in order and allocates nothing; a request the rest of
+// the workspace cannot hold fails with kErrWorkspace. deviceFree of the most
+// recent slice gives its space back (a slice freed out of order is reclaimed
+// when the scope ends). Inside a scope, downloads do not use the pinned
+// staging buffer and phase timing records no events.
+struct WorkspaceScope {
+  WorkspaceScope(void *base, size_t bytes);
+  ~WorkspaceScope();
+  WorkspaceScope(const WorkspaceScope &) = delete;
+  WorkspaceScope &operator=(const WorkspaceScope &) = delete;
+};
+bool inWorkspace();
+
 // Device memory through the library's cache of freed blocks (runtime.hip).
 // deviceFree takes the size given to deviceAlloc. cachedDeviceBytes: memory
 // held by the cache (free for the library's own use).

@@ -108,6 +108,19 @@ int cuhc_set_precision(int mode);
 int cuhc_linkage_device(const float *deviceX, int n, int d, int metricId, float metricParam,
                         int linkageId, int algorithmId, double *tree);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory while they run (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). cuhc_linkage_device for single linkage
+ * (linkageId 0, algorithmId 1) whose device buffers all come from one fixed
+ * 256 MiB array inside the library (one per GPU, held until the process
+ * ends): it calls no hipMalloc, hipFree, pinned-memory or other allocating
+ * routine. At most 65536 points of at most 64 features; larger inputs return
+ * -7 before any work. Same tree, statuses and precision/pruning modes as
+ * cuhc_linkage_device. Calls are serialised within the process; calls that
+ * overlap on one GPU by other means race on the shared array. */
+int cuhc_linkage_device_noalloc(const float *deviceX, int n, int d, int metricId,
+                                float metricParam, int linkageId, int algorithmId, double *tree);
+
 /* cuhc_hdbscan_tree with X in device memory; tree and core are host memory. */
 int cuhc_hdbscan_tree_device(const float *deviceX, int n, int d, int metricId, float metricParam,
                              int minSamples, float alpha, int coreMethod, double *tree,
