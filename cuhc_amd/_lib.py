@@ -55,6 +55,11 @@ _declare("cuhc_linkage_device", _int, ctypes.c_void_p, _int, _int, _int, _float,
          _f64p)
 _declare("cuhc_linkage_device_noalloc", _int, ctypes.c_void_p, _int, _int, _int, _float, _int,
          _int, _f64p)
+_declare("cuhc_exclusive_scan", _int, ctypes.c_void_p, ctypes.c_void_p, _int)
+_declare("cuhc_radix_sort_pairs_u32", _int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+         ctypes.c_void_p, _int, _int)
+_declare("cuhc_radix_sort_pairs_u64", _int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+         ctypes.c_void_p, _int, _int)
 _declare("cuhc_hdbscan_tree_device", _int, ctypes.c_void_p, _int, _int, _int, _float, _int,
          _float, _int, _f64p, _f32p)
 _declare("cuhc_device_alloc", _int, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_void_p))
@@ -67,7 +72,7 @@ _declare("cuhc_read_timings", None, _f64p, _i64p, _i64p, _int)
 _declare("cuhc_reset_timings", None)
 _declare("cuhc_release", None)
 _declare("cuhc_mixed_stats", None, _u64p, _u64p, _u64p, _u64p)
-_declare("cuhc_learned_stats", None, _u64p, _u64p, _u64p, _u64p, _u64p)
+_declare("cuhc_learned_stats", None, _u64p, _u64p, _u64p, _u64p, _u64p, _u64p, _u64p)
 _declare("cuhc_set_learned_training", _int, _int)
 _declare("cuhc_mlp_param_count", _int)
 _declare("cuhc_mlp_gradient", _int, _f32p, _f32p, _f32p, _int, _f32p, _f32p)
@@ -116,10 +121,10 @@ def mixed_stats():
 
 def learned_stats():
     """Counters of the learned-radius core distances since the last call (then reset)."""
-    vals = [ctypes.c_ulonglong() for _ in range(5)]
+    vals = [ctypes.c_ulonglong() for _ in range(7)]
     lib.cuhc_learned_stats(*[ctypes.byref(v) for v in vals])
-    return dict(zip(["proposals", "hits", "fallbacks", "candidates", "tiles"],
-                    [v.value for v in vals]))
+    return dict(zip(["proposals", "hits", "fallbacks", "candidates", "tiles", "settled_rows",
+                     "offered_rows"], [v.value for v in vals]))
 
 
 def set_timing(on):

@@ -37,7 +37,7 @@ DEVICE = [
 ]
 
 SOURCES = [
-    "runtime.hip", "distance.hip", "boruvka.hip", "core.hip", "dense.hip", "ward.hip",
+    "runtime.hip", "primitives.hip", "distance.hip", "boruvka.hip", "core.hip", "dense.hip", "ward.hip",
     "mfma.hip", "mlp.hip", "learned.hip", "edge_sort.hip", "contract.hip", "capi.hip", "noalloc.hip", "host_tree.cpp", "extract.cpp",
 ]
 

@@ -100,6 +100,17 @@ int boruvkaMst(const float *X, int n, int d, MetricSpec m, WeightSpec w, const M
     s(dSizes.alloc(n));
     s(dSkip.alloc(1));
   }
+  // Rows settled from the learned candidate lists (mutual reachability,
+  // alpha = 1) are closed to the walk.
+  const bool useLists = opt.lists && opt.prune && w.kind == kWeightMReach && w.alpha == 1.0f;
+  DevArray<unsigned char> dClosed;
+  DevArray<unsigned long long> dListCounters;
+  if (useLists) {
+    s(dClosed.alloc(n));
+    s(dListCounters.alloc(64 * 16));
+    if (s.ok())
+      s(fill(dListCounters, 0, dListCounters.bytes()));
+  }
   if (s.ok())
     s(resetComponents(dComp, n));
   MixedPass *mixed = nullptr;
@@ -124,8 +135,11 @@ int boruvkaMst(const float *X, int n, int d, MetricSpec m, WeightSpec w, const M
       if (pruneThisPass) {
         double share = 1.0;
         s(majorityComponent(dComp, n, numComponents, dSizes, dSkip));
+        if (useLists)
+          s(settleFromLists(*opt.lists, n, dComp, opt.devOrig, w.core, dSkip, dNearest,
+                            dCompBest, dClosed, dListCounters));
         s(nearestOtherWalk(X, n, d, m, w, dComp, opt.devOrig, dCompBest, dNearest, &share,
-                           opt.stats, 1, dSkip));
+                           opt.stats, 1, dSkip, useLists ? dClosed.get() : nullptr));
         pruneThisPass = share <= kPruneGiveUpShare;
       } else {
         s(nearestOtherSymmetric(X, n, d, m, w, dComp, opt.devOrig, dNearest));
@@ -143,6 +157,17 @@ int boruvkaMst(const float *X, int n, int d, MetricSpec m, WeightSpec w, const M
   }
   if (mixed)
     mixedPassDestroy(mixed);
+  if (useLists && s.ok()) {
+    std::vector<unsigned long long> ctr(64 * 16);
+    s(download(ctr.data(), dListCounters, dListCounters.bytes()));
+    LearnedStats st;
+    for (int k = 0; k < 64; ++k) {
+      st.settled += ctr[k * 16];
+      st.considered += ctr[k * 16 + 1];
+    }
+    if (s.ok())
+      learnedStatsAdd(st);
+  }
   return s.code;
 }
 
@@ -201,9 +226,18 @@ int hdbscanTree(const DevicePoints &pts, int n, int d, MetricSpec m, int minSamp
   DevArray<float> core;
   Status s;
   s(core.alloc(n));
-  if (s.ok())
-    s(coreDistances(pts.data(), n, d, m, minSamples, core, coreMethod,
-                    prune && pts.ordered()));
+  // The learned method's candidate lists also settle Boruvka rows (pruned
+  // passes, mutual reachability with alpha = 1, a box metric).
+  RadiusLists lists;
+  const bool keepLists = coreMethod == kCoreLearned && prune && pts.ordered() && alpha == 1.0f &&
+                         metricHasBoxBound(m.id);
+  if (s.ok()) {
+    if (keepLists)
+      s(coreLearned(pts.data(), n, d, m, minSamples, core, true, &lists));
+    else
+      s(coreDistances(pts.data(), n, d, m, minSamples, core, coreMethod,
+                      prune && pts.ordered()));
+  }
   // Pruned passes: reorder by core-distance class (the tree does not depend
   // on the order).
   const float *X = pts.data();
@@ -233,12 +267,25 @@ int hdbscanTree(const DevicePoints &pts, int n, int d, MetricSpec m, int minSamp
         devOrig = sOrig;
         order = &stratified;
       }
+      if (s.ok() && keepLists) {
+        // List rows stay in the old order (rowMap = new -> old); partners
+        // are renumbered to the new positions.
+        DevArray<int> oldToNew;
+        s(oldToNew.alloc(n));
+        if (s.ok())
+          s(inverseOrder(sMap, n, oldToNew));
+        if (s.ok())
+          s(remapRadiusLists(lists, n, oldToNew));
+        lists.rowMap = sMap;
+        s(sync());
+      }
     }
   }
   MstOptions opt;
   opt.prune = prune && pts.ordered();
   opt.devOrig = devOrig;
   opt.stats = stats;
+  opt.lists = keepLists && lists.pos.get() ? &lists : nullptr;
   WeightSpec w;
   w.kind = kWeightMReach;
   w.core = stratified.empty() ? core.get() : sCore.get();

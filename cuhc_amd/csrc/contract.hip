@@ -15,7 +15,7 @@
 // memory until the tree is assembled.
 #include "mst.hpp"
 
-#include <hipcub/hipcub.hpp>
+#include "primitives.hpp"
 
 namespace cuhc {
 
@@ -187,10 +187,8 @@ int Contraction::alloc(int points) {
   s(edges.u.alloc(std::max(1, n - 1)));
   s(edges.v.alloc(std::max(1, n - 1)));
   s(edges.w.alloc(std::max(1, n - 1)));
-  size_t bytes = 0;
-  s((int)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (int *)nullptr, (int *)nullptr, n + 1,
-                                          stream()));
-  s(scanTemp.alloc(std::max<size_t>(bytes, 1)));
+  const size_t bytes = scanTempBytes(n + 1);
+  s(scanTemp.alloc(bytes));
   scanBytes = bytes;
   s(fill(counters, 0, sizeof(int)));
   edges.count = 0;
@@ -248,9 +246,7 @@ int contractRound(Contraction &ct, const unsigned long long *nearest, int *comp,
     return kErrNonFinite; // hooks with a cycle: inconsistent keys
   hipLaunchKernelGGL(rootFlagsKernel, gridFor(numComp + 1), dim3(256), 0, stream(),
                      ct.root.get(), numComp, ct.flags.get());
-  size_t bytes = ct.scanBytes;
-  s((int)hipcub::DeviceScan::ExclusiveSum(ct.scanTemp.get(), bytes, ct.flags.get(),
-                                          ct.newId.get(), numComp + 1, stream()));
+  s(exclusiveScan(ct.flags.get(), ct.newId.get(), numComp + 1, ct.scanTemp.get()));
   hipLaunchKernelGGL(relabelKernel, gridFor(n), dim3(256), 0, stream(), comp, n, ct.root.get(),
                      ct.newId.get());
   s((int)hipGetLastError());

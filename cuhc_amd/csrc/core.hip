@@ -21,9 +21,13 @@ namespace cuhc {
 
 namespace {
 
+// Column tile boxes (low/high per tile, numTiles > 0 turns pruning on), and
+// for gathered rows the boxes of their row tiles (rowLow/rowHigh; null: the
+// rows are the points themselves and share the column tiles' boxes).
 struct CoreBoxes {
   const float *low = nullptr, *high = nullptr;
   int numTiles = 0;
+  const float *rowLow = nullptr, *rowHigh = nullptr;
 };
 
 DEV int walkTileCore(int rowTile, int step, int numTiles) {
@@ -83,11 +87,14 @@ __global__ void __launch_bounds__(256)
     lists[i] = INFINITY;
   const bool prune = boxes.numTiles > 0;
   float *rowBox = lists + T::DIM * k;
-  if (prune)
+  if (prune) {
+    const float *rl = boxes.rowLow ? boxes.rowLow : boxes.low;
+    const float *rh = boxes.rowHigh ? boxes.rowHigh : boxes.high;
     for (int f = threadIdx.x; f < d; f += T::THREADS) {
-      rowBox[f] = boxes.low[(size_t)rowTile * d + f];
-      rowBox[d + f] = boxes.high[(size_t)rowTile * d + f];
+      rowBox[f] = rl[(size_t)rowTile * d + f];
+      rowBox[d + f] = rh[(size_t)rowTile * d + f];
     }
+  }
   if (threadIdx.x == 0) {
     largestKth = INFINITY; // every list starts empty
     inserted = 0;
@@ -97,8 +104,11 @@ __global__ void __launch_bounds__(256)
   const int numTiles = (n + T::DIM - 1) / T::DIM;
   const int lane = threadIdx.x & 63;
   const unsigned long long groupMask = 0xffffull << (lane & 48);
-  // Listed rows have no home tile: the walk starts anywhere and covers all.
-  const int homeTile = rowsX ? (int)(blockIdx.x % numTiles) : rowTile;
+  // Listed rows have no home tile: the walk starts anywhere and covers all;
+  // with pruning (rows listed in point order) it starts at the column tile of
+  // the tile's first row, which is near the others.
+  const int homeTile =
+      rowsX ? (prune ? rowPos[row0] / T::DIM : (int)(blockIdx.x % numTiles)) : rowTile;
 
   // A step is skipped when the box bound between the row tile and its column
   // tile exceeds the largest k-th distance of the tile's rows: wave 0 computes
@@ -380,10 +390,13 @@ int coreLocal(const float *X, int n, int d, MetricSpec m, int k, int stepLimit, 
 namespace {
 
 // Exact core distances of listed rows with the fused top-k lists (k <= 64):
-// the rows are gathered, every column tile is visited (no pruning: listed rows
-// are scattered).
+// the rows are gathered. Without pruning every column tile is visited; with
+// it (points in k-d order, rows listed in ascending position, a box metric)
+// each tile of 128 gathered rows gets its own box and skips the column tiles
+// whose bound exceeds its largest k-th distance, as the tiles of the points
+// themselves do.
 int coreForRowsFused(const float *X, int n, int d, MetricSpec m, int k, const int *rows,
-                     int numRows, float *core) {
+                     int numRows, float *core, bool prune) {
   DevArray<float> rowData;
   Status s;
   s(rowData.alloc((size_t)numRows * d));
@@ -396,12 +409,29 @@ int coreForRowsFused(const float *X, int n, int d, MetricSpec m, int k, const in
     auto run = [&](auto fill) {
       using T = typename TileFor<Metric, decltype(fill)::value>::type;
       const int rowTiles = (numRows + T::DIM - 1) / T::DIM;
-      const size_t smem = (size_t)T::DIM * k * sizeof(float);
+      size_t smem = (size_t)T::DIM * k * sizeof(float);
+      CoreBoxes boxes;
+      DevArray<float> low, high, rowLow, rowHigh;
+      Status st;
+      if (prune && Metric::kBoxBound) {
+        const int tiles = (n + T::DIM - 1) / T::DIM;
+        st(low.alloc((size_t)tiles * d));
+        st(high.alloc((size_t)tiles * d));
+        st(rowLow.alloc((size_t)rowTiles * d));
+        st(rowHigh.alloc((size_t)rowTiles * d));
+        if (!st.ok())
+          return st.code;
+        hipLaunchKernelGGL(coreBoxesKernel, dim3(tiles), dim3(64), 0, stream(), X, n, d, T::DIM,
+                           low.get(), high.get());
+        hipLaunchKernelGGL(coreBoxesKernel, dim3(rowTiles), dim3(64), 0, stream(),
+                           rowData.get(), numRows, d, T::DIM, rowLow.get(), rowHigh.get());
+        boxes = CoreBoxes{low, high, tiles, rowLow, rowHigh};
+        smem += 2 * d * sizeof(float);
+      }
       PhaseTimer timer(kPhaseCore);
       hipLaunchKernelGGL((kthNearestFusedKernel<T, Metric>), dim3(rowTiles), dim3(256), smem,
-                         stream(), X, n, d, metric, k, core, CoreBoxes{}, 0x7fffffff,
+                         stream(), X, n, d, metric, k, core, boxes, 0x7fffffff,
                          rowData.get(), numRows, rows, nullptr);
-      Status st;
       st((int)hipGetLastError());
       st(sync());
       return st.code;
@@ -415,11 +445,11 @@ int coreForRowsFused(const float *X, int n, int d, MetricSpec m, int k, const in
 } // namespace
 
 int coreForRows(const float *X, int n, int d, MetricSpec m, int k, const int *rows, int numRows,
-                float *core) {
+                float *core, bool prune) {
   if (numRows <= 0)
     return 0;
   if (k <= kMaxFusedK)
-    return coreForRowsFused(X, n, d, m, k, rows, numRows, core);
+    return coreForRowsFused(X, n, d, m, k, rows, numRows, core, prune);
   const int chunkRows = radixChunkRows(n, numRows, 1);
   DevArray<float> chunk, rowData;
   Status s;

@@ -466,7 +466,8 @@ __global__ void __launch_bounds__(256)
                            unsigned int *__restrict__ compBest,
                            unsigned long long *__restrict__ nearest,
                            unsigned long long *tilesComputed, int rowTileStride,
-                           int computeSteps, const int *__restrict__ skipComp) {
+                           int computeSteps, const int *__restrict__ skipComp,
+                           const unsigned char *__restrict__ closedRows) {
   constexpr int TR = T::TR;
   constexpr int kBatch = 64;
   __shared__ __align__(16) TileSmem<T> sm;
@@ -488,10 +489,20 @@ __global__ void __launch_bounds__(256)
   const int skip = skipComp ? *skipComp : -1;
   if (skip >= 0 && boxes.minComp[rowTile] == skip && boxes.maxComp[rowTile] == skip)
     return;
+  // A row is open unless it lies past the end, in the skipped component, or
+  // is closed (its exact answer is already in nearest[]).
+  auto isOpen = [&](int row) {
+    return row < n && comp[row] != skip && !(closedRows && closedRows[row]);
+  };
+  int anyOpen = 0;
   for (int i = threadIdx.x; i < T::DIM; i += T::THREADS) {
     rowBest[i] = kEmptyKey;
-    rowNeed[i] = row0 + i < n && comp[row0 + i] != skip ? INFINITY : -1.0f;
+    const bool open = isOpen(row0 + i);
+    rowNeed[i] = open ? INFINITY : -1.0f;
+    anyOpen |= open;
   }
+  if (!__syncthreads_or(anyOpen))
+    return; // uniform: every row of the tile is settled
   for (int k = threadIdx.x; k < d; k += T::THREADS) {
     rowBox[k] = boxes.low[(size_t)rowTile * d + k];
     rowBox[d + k] = boxes.high[(size_t)rowTile * d + k];
@@ -499,8 +510,8 @@ __global__ void __launch_bounds__(256)
   // Wave 0 owns rows lane and lane + 64 for the bound: their components and
   // the components' best known edges (refreshed once per batch).
   const int ra = row0 + (int)(threadIdx.x & 63), rb = ra + 64;
-  const bool open0 = threadIdx.x < 64 && ra < n && comp[ra] != skip;
-  const bool open1 = threadIdx.x < 64 && rb < n && comp[rb] != skip;
+  const bool open0 = threadIdx.x < 64 && isOpen(ra);
+  const bool open1 = threadIdx.x < 64 && isOpen(rb);
   const int comp0 = open0 ? comp[ra] : 0, comp1 = open1 ? comp[rb] : 0;
   float cap0 = INFINITY, cap1 = INFINITY;
   auto capOf = [&](unsigned int bits) { return bits == 0xffffffffu ? INFINITY : floatOf(bits); };
@@ -518,7 +529,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int r = 0; r < TR; ++r) {
     const int row = row0 + T::rowInTile(r);
-    rowComp[r] = row < n && comp[row] != skip ? comp[row] : -2;
+    rowComp[r] = isOpen(row) ? comp[row] : -2;
     rowVal[r] = weight.value(row < n ? row : 0);
   }
   unsigned long long computed = 0;
@@ -937,7 +948,7 @@ template <class T, class Metric, class Weight>
 int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &weight,
             const int *comp, const int *orig, unsigned int *compBest,
             unsigned long long *nearest, double *share, PassStats *stats, int stride,
-            const int *skipComp) {
+            const int *skipComp, const unsigned char *closedRows) {
   const int tiles = (n + T::DIM - 1) / T::DIM;
   DeviceBoxes boxes;
   DevArray<unsigned long long> counter;
@@ -956,7 +967,7 @@ int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &we
     hipLaunchKernelGGL((nearestOtherWalkKernel<T, Metric, Weight>), dim3(sampled), dim3(256),
                        2 * d * sizeof(float), stream(), X, n, d, metric, weight, comp, orig,
                        boxes.view, compBest, nearest, counter.get(), stride,
-                       stride > 1 ? kEstimateSteps : 0x7fffffff, skipComp);
+                       stride > 1 ? kEstimateSteps : 0x7fffffff, skipComp, closedRows);
     s((int)hipGetLastError());
   }
   unsigned long long computed = 0;
@@ -975,7 +986,8 @@ int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &we
 
 int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, const int *comp,
                      const int *orig, unsigned int *compBest, unsigned long long *nearest,
-                     double *share, PassStats *stats, int stride, const int *skipComp) {
+                     double *share, PassStats *stats, int stride, const int *skipComp,
+                     const unsigned char *closedRows) {
   if (!metricHasBoxBound(m.id))
     return kErrInvalidMetric;
   if (w.kind == kWeightWard) {
@@ -984,7 +996,7 @@ int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, c
     return withFill(d, [&](auto fill) {
       using T = Tile<8, decltype(fill)::value>;
       return runWalk<T>(X, n, d, SqEuclidean{}, WardWeight{w.size}, comp, orig, compBest,
-                        nearest, share, stats, stride, skipComp);
+                        nearest, share, stats, stride, skipComp, closedRows);
     });
   }
   return withMetric(m, [&](auto metric) {
@@ -996,7 +1008,7 @@ int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, c
         return withFill(d, [&](auto fill) {
           using T = Tile<8, decltype(fill)::value>;
           return runWalk<T>(X, n, d, metric, weight, comp, orig, compBest, nearest, share,
-                            stats, stride, skipComp);
+                            stats, stride, skipComp, closedRows);
         });
       });
     }

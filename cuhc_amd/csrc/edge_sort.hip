@@ -5,7 +5,7 @@
 // the same weight) are ordered by the larger endpoint on the host afterwards.
 #include "mst.hpp"
 
-#include <hipcub/hipcub.hpp>
+#include "primitives.hpp"
 
 #include <algorithm>
 
@@ -73,21 +73,15 @@ int treeFromDeviceEdges(const DeviceEdges &edges, int n, double *tree) {
     s(idx.alloc(m));
     s(idxOut.alloc(m));
     s(packed.alloc(m));
-    size_t tempBytes = 0;
-    s((int)hipcub::DeviceRadixSort::SortPairs(nullptr, tempBytes, (unsigned long long *)nullptr,
-                                              (unsigned long long *)nullptr,
-                                              (unsigned int *)nullptr, (unsigned int *)nullptr, m,
-                                              0, 32 + uBits, stream()));
-    s(temp.alloc(std::max<size_t>(tempBytes, 1)));
+    s(temp.alloc(sortTempBytes64(m)));
     if (!s.ok())
       return s.code;
     const dim3 grid((m + 255) / 256);
     hipLaunchKernelGGL(edgeKeysKernel, grid, dim3(256), 0, stream(), edges.u.get(),
                        edges.w.get(), m, uBits, keys.get(), idx.get());
     s((int)hipGetLastError());
-    s((int)hipcub::DeviceRadixSort::SortPairs(temp.get(), tempBytes, keys.get(), keysOut.get(),
-                                              idx.get(), idxOut.get(), m, 0, 32 + uBits,
-                                              stream()));
+    s(radixSortPairs(keys.get(), keysOut.get(), idx.get(), idxOut.get(), m, 32 + uBits,
+                     temp.get()));
     hipLaunchKernelGGL(edgeGatherKernel, grid, dim3(256), 0, stream(), edges.u.get(),
                        edges.v.get(), edges.w.get(), idxOut.get(), m, packed.get());
     s((int)hipGetLastError());

@@ -132,7 +132,8 @@ __global__ void __launch_bounds__(256)
     radiusGatherKernel(const float *__restrict__ X, int n, int d, Metric metric,
                        const float *__restrict__ radius, const float *__restrict__ low,
                        const float *__restrict__ high, float *__restrict__ cand,
-                       int *__restrict__ count, unsigned long long *tilesComputed) {
+                       int *__restrict__ candPos, int *__restrict__ count,
+                       unsigned long long *tilesComputed) {
   constexpr int TR = T::TR;
   __shared__ __align__(16) TileSmem<T> sm;
   __shared__ int cnt[T::DIM];
@@ -202,8 +203,11 @@ __global__ void __launch_bounds__(256)
           const float val = col == row ? 0.0f : fmaxf(metric.finish(v[r][c]), 0.0f);
           if (val <= rRow[r]) {
             const int pos = atomicAdd(&cnt[rit], 1);
-            if (pos < kCandPerRow)
+            if (pos < kCandPerRow) {
               cand[(size_t)row * kCandPerRow + pos] = val;
+              if (candPos)
+                candPos[(size_t)row * kCandPerRow + pos] = col;
+            }
           }
         }
       }
@@ -266,6 +270,68 @@ __global__ void __launch_bounds__(256)
     core[row] = __uint_as_float(prefix);
 }
 
+__global__ void remapListsKernel(int *__restrict__ pos, const int *__restrict__ count, int n,
+                                 int cap, const int *__restrict__ oldToNew) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * cap)
+    return;
+  const int row = (int)(e / cap), slot = (int)(e - (size_t)row * cap);
+  if (slot < min(count[row], cap))
+    pos[e] = oldToNew[pos[e]];
+}
+
+// One wave per row (see settleFromLists).
+__global__ void __launch_bounds__(256)
+    settleFromListsKernel(const float *__restrict__ dist, const int *__restrict__ pos,
+                          const int *__restrict__ count, const float *__restrict__ radius,
+                          const int *__restrict__ rowMap, int cap, int n,
+                          const int *__restrict__ comp, const int *__restrict__ orig,
+                          const float *__restrict__ core, const int *__restrict__ skipComp,
+                          unsigned long long *__restrict__ nearest,
+                          unsigned int *__restrict__ compBest, unsigned char *__restrict__ closed,
+                          unsigned long long *counters) {
+  const int lane = threadIdx.x & 63;
+  const int p = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (p >= n)
+    return;
+  const int c = comp[p];
+  const int skip = skipComp ? *skipComp : -1;
+  bool settled = false;
+  if (c != skip) {
+    const int row = rowMap ? rowMap[p] : p;
+    const int cnt = count[row];
+    if (cnt <= cap) {
+      const MReachWeightT<false> weight{core, 1.0f};
+      const float ci = core[p];
+      unsigned long long best = kEmptyKey;
+      for (int e = lane; e < cnt; e += 64) {
+        const int j = pos[(size_t)row * cap + e];
+        if (comp[j] == c)
+          continue;
+        const float w = weight(dist[(size_t)row * cap + e], ci, core[j]);
+        if (w != INFINITY)
+          best = minKey(best, packKey(w, orig ? (unsigned int)orig[j] : (unsigned int)j));
+      }
+#pragma unroll
+      for (int m = 32; m > 0; m >>= 1)
+        best = minKey(best, shflXorKey(best, m));
+      settled = best != kEmptyKey && keyWeight(best) <= radius[row];
+      if (settled && lane == 0) {
+        nearest[p] = best;
+        if (compBest)
+          atomicMin(&compBest[c], (unsigned int)(best >> 32));
+      }
+    }
+    if (lane == 0) // striped over 64 lines: one address would serialise every row
+      atomicAdd(&counters[(p & 63) * 16 + 1], 1ull);
+  }
+  if (lane == 0) {
+    closed[p] = settled ? 1 : 0;
+    if (settled)
+      atomicAdd(&counters[(p & 63) * 16], 1ull);
+  }
+}
+
 template <class F> int withBoxMetric(MetricSpec m, F &&f) {
   switch (m.id) {
   case kEuclidean: return f(Euclidean{});
@@ -286,17 +352,44 @@ void learnedStatsAdd(const LearnedStats &s) {
   gLearned.fallbacks += s.fallbacks;
   gLearned.candidates += s.candidates;
   gLearned.tiles += s.tiles;
+  gLearned.settled += s.settled;
+  gLearned.considered += s.considered;
 }
 
-int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, bool spatial) {
-  (void)spatial; // exact in any order; proposals are only sharp for k-d ordered points
+int remapRadiusLists(RadiusLists &lists, int n, const int *oldToNew) {
+  if (!lists.pos.get() || lists.cap == 0)
+    return 0;
+  const size_t total = (size_t)n * lists.cap;
+  hipLaunchKernelGGL(remapListsKernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     stream(), lists.pos.get(), lists.count.get(), n, lists.cap, oldToNew);
+  return (int)hipGetLastError();
+}
+
+int settleFromLists(const RadiusLists &lists, int n, const int *comp, const int *orig,
+                    const float *core, const int *skipComp, unsigned long long *nearest,
+                    unsigned int *compBest, unsigned char *closed, unsigned long long *counters) {
+  hipLaunchKernelGGL(settleFromListsKernel, dim3((n + 3) / 4), dim3(256), 0, stream(),
+                     lists.dist.get(), lists.pos.get(), lists.count.get(), lists.radius.get(),
+                     lists.rowMap, lists.cap, n, comp, orig, core, skipComp, nearest, compBest,
+                     closed, counters);
+  return (int)hipGetLastError();
+}
+
+int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, bool spatial,
+                RadiusLists *keep) {
+  // Exact in any order; the proposals are only sharp, and the fallback search
+  // only pruned, for points in k-d order (spatial).
   if (!metricHasBoxBound(m.id))
     return coreDistances(X, n, d, m, k, core, kCoreAuto, false);
   constexpr int DIM = 128;
   const int tiles = (n + DIM - 1) / DIM;
   Status s;
-  DevArray<float> local1, local3, low, high, centroid, diag, feats, params, radius, pred, cand;
-  DevArray<int> count, fallback, numFallback;
+  DevArray<float> local1, local3, low, high, centroid, diag, feats, params, pred;
+  DevArray<int> fallback, numFallback;
+  RadiusLists own;
+  RadiusLists &lists = keep ? *keep : own;
+  DevArray<float> &radius = lists.radius, &cand = lists.dist;
+  DevArray<int> &count = lists.count;
   DevArray<unsigned long long> counters;
   s(local1.alloc(n));
   s(local3.alloc(n));
@@ -403,6 +496,9 @@ int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, 
   hipLaunchKernelGGL(radiusKernel, dim3((n + 255) / 256), dim3(256), 0, stream(), pred.get(), n,
                      (float)tsd, (float)(tm + margin), radius.get());
   s(cand.alloc((size_t)n * kCandPerRow));
+  if (keep)
+    s(lists.pos.alloc((size_t)n * kCandPerRow));
+  lists.cap = kCandPerRow;
   s(fill(numFallback, 0, sizeof(int)));
   s(fill(counters, 0, counters.bytes()));
   if (!s.ok())
@@ -413,7 +509,8 @@ int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, 
       using T = Tile<8, decltype(fill)::value>;
       hipLaunchKernelGGL((radiusGatherKernel<T, Metric>), dim3(tiles), dim3(256),
                          2 * d * sizeof(float), stream(), X, n, d, metric, radius.get(),
-                         low.get(), high.get(), cand.get(), count.get(), counters.get());
+                         low.get(), high.get(), cand.get(), lists.pos.get(), count.get(),
+                         counters.get());
       return (int)hipGetLastError();
     };
     return d <= 8 ? run(std::integral_constant<int, 8>{}) : run(std::integral_constant<int, 32>{});
@@ -424,8 +521,16 @@ int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, 
   s((int)hipGetLastError());
   int nf = 0;
   s(download(&nf, numFallback, sizeof(int)));
-  if (s.ok() && nf > 0)
-    s(coreForRows(X, n, d, m, k, fallback, nf, core));
+  if (s.ok() && nf > 0) {
+    // The failed rows in ascending position (k-d order): tiles of them are
+    // spatially compact enough for the pruned search.
+    std::vector<int> rowsFailed(nf);
+    s(download(rowsFailed.data(), fallback, nf * sizeof(int)));
+    std::sort(rowsFailed.begin(), rowsFailed.end());
+    s(upload(fallback, rowsFailed.data(), nf * sizeof(int)));
+    if (s.ok())
+      s(coreForRows(X, n, d, m, k, fallback, nf, core, spatial));
+  }
   std::vector<unsigned long long> ctr(counters.count);
   s(download(ctr.data(), counters, counters.bytes()));
   LearnedStats st;
@@ -445,13 +550,18 @@ int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, 
 
 extern "C" void cuhc_learned_stats(unsigned long long *proposals, unsigned long long *hits,
                                    unsigned long long *fallbacks, unsigned long long *candidates,
-                                   unsigned long long *tiles) {
+                                   unsigned long long *tiles, unsigned long long *settled,
+                                   unsigned long long *considered) {
   std::lock_guard<std::mutex> lock(cuhc::gLearnedMutex);
   *proposals = cuhc::gLearned.proposals;
   *hits = cuhc::gLearned.hits;
   *fallbacks = cuhc::gLearned.fallbacks;
   *candidates = cuhc::gLearned.candidates;
   *tiles = cuhc::gLearned.tiles;
+  if (settled)
+    *settled = cuhc::gLearned.settled;
+  if (considered)
+    *considered = cuhc::gLearned.considered;
   cuhc::gLearned = cuhc::LearnedStats{};
 }
 

@@ -47,7 +47,7 @@
 // finite), which lets fminf map to v_min/v_min3 without canonicalisation.
 #include "mst.hpp"
 
-#include <hipcub/hipcub.hpp>
+#include "primitives.hpp"
 
 #include <algorithm>
 #include <cfloat>
@@ -1047,13 +1047,10 @@ MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
   s(mp->smallSlots.alloc(n + 1));
   s(mp->bigBase.alloc(n + 1));
   s(mp->smallBase.alloc(n + 1));
-  s((int)hipcub::DeviceRadixSort::SortPairs(nullptr, mp->sortBytes, (unsigned int *)nullptr,
-                                            (unsigned int *)nullptr, (int *)nullptr,
-                                            (int *)nullptr, n, 0, 32, stream()));
-  s((int)hipcub::DeviceScan::ExclusiveSum(nullptr, mp->scanBytes, (int *)nullptr, (int *)nullptr,
-                                          n + 1, stream()));
-  s(mp->sortTemp.alloc(std::max<size_t>(mp->sortBytes, 1)));
-  s(mp->scanTemp.alloc(std::max<size_t>(mp->scanBytes, 1)));
+  mp->sortBytes = sortTempBytes32(n);
+  mp->scanBytes = scanTempBytes(n + 1);
+  s(mp->sortTemp.alloc(mp->sortBytes));
+  s(mp->scanTemp.alloc(mp->scanBytes));
   if (s.ok())
     hipLaunchKernelGGL(positionsKernel, dim3((n + 255) / 256), dim3(256), 0, stream(),
                        mp->positions.get(), n);
@@ -1122,12 +1119,10 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   int bits = 1;
   while (bits < 31 && (numComponents - 1) >> bits)
     ++bits;
-  size_t bytes = mp->sortBytes;
   // Component labels are non-negative: sorted as unsigned keys.
-  s((int)hipcub::DeviceRadixSort::SortPairs(
-      mp->sortTemp.get(), bytes, reinterpret_cast<const unsigned int *>(comp),
-      reinterpret_cast<unsigned int *>(mp->sortedComp.get()), mp->positions.get(),
-      mp->perm.get(), n, 0, bits, stream()));
+  s(radixSortPairs(reinterpret_cast<const unsigned int *>(comp),
+                   reinterpret_cast<unsigned int *>(mp->sortedComp.get()), mp->positions.get(),
+                   mp->perm.get(), n, bits, mp->sortTemp.get()));
   const dim3 grid((n + 255) / 256);
   hipLaunchKernelGGL(componentRangesKernel, grid, dim3(256), 0, stream(), mp->sortedComp.get(), n,
                      mp->compStart.get(), mp->compEnd.get());
@@ -1142,18 +1137,14 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   hipLaunchKernelGGL(staleRowsKernel, dim3((n + 1 + 255) / 256), dim3(256), 0, stream(),
                      mp->perm.get(), comp, mp->topk.get(), nearest, n, firstRound,
                      mp->largest.get(), mp->rowFlag.get());
-  bytes = mp->scanBytes;
-  s((int)hipcub::DeviceScan::ExclusiveSum(mp->scanTemp.get(), bytes, mp->rowFlag.get(),
-                                          mp->rowIndex.get(), n + 1, stream()));
+  s(exclusiveScan(mp->rowFlag.get(), mp->rowIndex.get(), n + 1, mp->scanTemp.get()));
   hipLaunchKernelGGL(componentSlotsKernel, dim3((numComponents + 1 + 255) / 256), dim3(256), 0,
                      stream(), mp->compStart.get(), mp->compEnd.get(), mp->rowIndex.get(),
                      numComponents, mp->bigSlots.get(), mp->smallSlots.get());
-  bytes = mp->scanBytes;
-  s((int)hipcub::DeviceScan::ExclusiveSum(mp->scanTemp.get(), bytes, mp->bigSlots.get(),
-                                          mp->bigBase.get(), numComponents + 1, stream()));
-  bytes = mp->scanBytes;
-  s((int)hipcub::DeviceScan::ExclusiveSum(mp->scanTemp.get(), bytes, mp->smallSlots.get(),
-                                          mp->smallBase.get(), numComponents + 1, stream()));
+  s(exclusiveScan(mp->bigSlots.get(), mp->bigBase.get(), numComponents + 1,
+                  mp->scanTemp.get()));
+  s(exclusiveScan(mp->smallSlots.get(), mp->smallBase.get(), numComponents + 1,
+                  mp->scanTemp.get()));
   s(fill(mp->rows, 0xff, mp->rows.bytes())); // every slot starts as padding (position -1)
   hipLaunchKernelGGL(rowListKernel, grid, dim3(256), 0, stream(), mp->perm.get(), comp,
                      mp->rowFlag.get(), mp->rowIndex.get(), mp->compStart.get(),

@@ -60,11 +60,29 @@ int contractRound(Contraction &ct, const unsigned long long *nearest, int *comp,
 // Sorts device edges into the global order and writes the scipy-format tree.
 int treeFromDeviceEdges(const DeviceEdges &edges, int n, double *tree);
 
+// The learned method's candidate lists, kept for the Boruvka rounds of
+// mutual reachability. List row r (the points' order at the gather) holds
+// count[r] entries: every point within radius[r] of point r (itself
+// included), as FP32 distances and partner positions; count[r] > cap means
+// the list overflowed and is incomplete. rowMap: position in the current
+// order -> list row (null: the same order).
+struct RadiusLists {
+  DevArray<float> dist;   // [n][cap]
+  DevArray<int> pos;      // [n][cap]
+  DevArray<int> count;    // [n]
+  DevArray<float> radius; // [n]
+  const int *rowMap = nullptr;
+  int cap = 0;
+};
+
 struct MstOptions {
   bool prune = false;          // points in spatial order; use the walk pass
   const int *devOrig = nullptr;  // position -> original index (device), null = identity
   PassStats *stats = nullptr;
   bool mixed = false;          // FP16 MFMA candidate pass (plain euclidean weights only)
+  // Learned-radius candidate lists (mutual reachability, alpha = 1, pruned
+  // passes): rows they settle skip the walk.
+  const RadiusLists *lists = nullptr;
 };
 
 // Exact MST of the complete graph over the points under the total order
@@ -85,18 +103,40 @@ int coreDistances(const float *X, int n, int d, MetricSpec m, int k, float *core
 // The k-th smallest over the first stepLimit walk steps only (own tile, then
 // neighbours): a local density feature.
 int coreLocal(const float *X, int n, int d, MetricSpec m, int k, int stepLimit, float *out);
-// Exact core distances of the listed points (device positions) against all points.
+// Exact core distances of the listed points (device positions) against all
+// points. prune: the points are in k-d order and the list ascends, so tiles
+// of listed rows get their own boxes and skip far column tiles (k <= 64).
 int coreForRows(const float *X, int n, int d, MetricSpec m, int k, const int *rows, int numRows,
-                float *core);
+                float *core, bool prune = false);
 // Exact core distances of every point of the listed 128-point row tiles
 // (host list), with the pruned walk when k <= kMaxFusedK and the metric has a
 // box bound, else through coreForRows.
 int coreForTiles(const float *X, int n, int d, MetricSpec m, int k, const int *hostTiles,
                  int numTiles, float *core);
-int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, bool spatial);
+// keep (nullable): receives the candidate lists.
+int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, bool spatial,
+                RadiusLists *keep = nullptr);
+
+// Partners renumbered for a new point order: pos = oldToNew[pos] (device).
+int remapRadiusLists(RadiusLists &lists, int n, const int *oldToNew);
+
+// Settles rows of a Boruvka round from the lists (mutual reachability with
+// alpha = 1, whose weight max(core_i, core_j, d_ij) is never below d_ij):
+// for row p, the smallest key (weight, orig[j]) over the listed partners j in
+// other components is its exact nearest-other-component key whenever its
+// weight is at most radius[p]: any partner missing from a complete list is
+// farther than the radius, so its weight is larger. Such rows get nearest[p]
+// = that key, their component's best known weight (compBest, nullable)
+// lowered to it, and closed[p] = 1; the others closed[p] = 0. Rows of the
+// skipped component (skipComp, nullable) stay open and untouched.
+// counters[0] += rows settled, counters[1] += rows considered (device).
+int settleFromLists(const RadiusLists &lists, int n, const int *comp, const int *orig,
+                    const float *core, const int *skipComp, unsigned long long *nearest,
+                    unsigned int *compBest, unsigned char *closed, unsigned long long *counters);
 
 struct LearnedStats {
   unsigned long long proposals = 0, hits = 0, fallbacks = 0, candidates = 0, tiles = 0;
+  unsigned long long settled = 0, considered = 0; // Boruvka rows answered from the lists
 };
 void learnedStatsAdd(const LearnedStats &s);
 

@@ -74,11 +74,12 @@ struct PassStats {
 // all the contraction uses. Returns the share of tile pairs computed in
 // *share. rowTileStride > 1 runs every stride-th row tile only (sampling).
 // skipComp (nullable, device): rows of that component (-1: none) search
-// nothing and keep their nearest[] entries.
+// nothing and keep their nearest[] entries; so do rows p with closedRows[p]
+// set (nullable, device), whose exact answers are already in nearest[].
 int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, const int *comp,
                      const int *orig, unsigned int *compBest, unsigned long long *nearest,
                      double *share, PassStats *stats, int rowTileStride = 1,
-                     const int *skipComp = nullptr);
+                     const int *skipComp = nullptr, const unsigned char *closedRows = nullptr);
 
 // *majority (device) = the component holding more than half of the n points,
 // or -1. sizes: numComp ints of scratch.

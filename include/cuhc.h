@@ -121,6 +121,18 @@ int cuhc_linkage_device(const float *deviceX, int n, int d, int metricId, float 
 int cuhc_linkage_device_noalloc(const float *deviceX, int n, int d, int metricId,
                                 float metricParam, int linkageId, int algorithmId, double *tree);
 
+/* The library's own device primitives, exposed as test hooks (no
+ * counterpart in the reference, whose scans are CUB's): exclusive prefix sum
+ * of ints, and stable LSD radix sorts of (key, value) pairs by the low `bits`
+ * bits of the keys (higher bits zero). Device arrays; inputs unchanged
+ * (in == out allowed for the scan); synchronous. Return a status. */
+int cuhc_exclusive_scan(const int *in, int *out, int count);
+int cuhc_radix_sort_pairs_u32(const unsigned int *keysIn, unsigned int *keysOut,
+                              const int *valuesIn, int *valuesOut, int count, int bits);
+int cuhc_radix_sort_pairs_u64(const unsigned long long *keysIn, unsigned long long *keysOut,
+                              const unsigned int *valuesIn, unsigned int *valuesOut, int count,
+                              int bits);
+
 /* cuhc_hdbscan_tree with X in device memory; tree and core are host memory. */
 int cuhc_hdbscan_tree_device(const float *deviceX, int n, int d, int metricId, float metricParam,
                              int minSamples, float alpha, int coreMethod, double *tree,
@@ -156,10 +168,13 @@ void cuhc_reset_timings(void);
 /* Learned-radius core distances (cuhc_hdbscan_tree coreMethod 3): counters
  * since the last call (then resets): radius proposals, proposals that held
  * (k neighbours inside the radius, list not overflowing), exact fallbacks, and
- * in-radius candidates gathered, distance tiles computed by the gather. */
+ * in-radius candidates gathered, distance tiles computed by the gather; then
+ * (nullable) Boruvka rows answered exactly from the kept candidate lists and
+ * rows offered to them, summed over the pruned rounds. */
 void cuhc_learned_stats(unsigned long long *proposals, unsigned long long *hits,
                         unsigned long long *fallbacks, unsigned long long *candidates,
-                        unsigned long long *tiles);
+                        unsigned long long *tiles, unsigned long long *settled,
+                        unsigned long long *considered);
 /* Adam steps used to train the radius network per call (default 300; 0 keeps
  * the untrained initialisation — a test hook that measures what training
  * gains). Returns -2 for negative values. */

@@ -177,3 +177,45 @@ def test_trained_network_gains_over_untrained():
     assert trained["hits"] >= 0.95 * len(X)
     assert trained["fallbacks"] < 0.5 * untrained["fallbacks"]
     assert trained["candidates"] < untrained["candidates"]
+
+
+@pytest.mark.parametrize("name", ["blobs_d8_noise", "blobs_d2", "duplicates", "mixed_scales"])
+@pytest.mark.parametrize("k", [5, 25])
+def test_learned_lists_settle_boruvka_rows(name, k):
+    # With pruned passes the learned method keeps its candidate lists, and a
+    # Boruvka row whose best listed partner outside its component lies within
+    # the radius is answered from the list and closed to the walk; the other
+    # rows are searched. The tree must equal the oracle's bit for bit, with
+    # both kinds of rows present.
+    X = DATA[name]
+    _lib.set_pruning("always")
+    try:
+        _lib.learned_stats()
+        Z, core = hdbscan_tree(X, k, core_method="learned")
+        st = _lib.learned_stats()
+    finally:
+        _lib.set_pruning("auto")
+    Zo, co = oracle.hdbscan_tree(X, k)
+    np.testing.assert_array_equal(core, co)
+    np.testing.assert_array_equal(Z, Zo)
+    print(name, k, st)
+    assert st["settled_rows"] > 0
+    assert st["settled_rows"] < st["offered_rows"]
+
+
+@pytest.mark.parametrize("alpha", [0.7, 1.6])
+def test_learned_lists_off_for_scaled_reachability(alpha):
+    # alpha != 1: the weight max(core_i, core_j, d / alpha) can fall below the
+    # distance, so the lists do not certify rows; the tree is still exact.
+    X = DATA["blobs_d8_noise"]
+    _lib.set_pruning("always")
+    try:
+        _lib.learned_stats()
+        Z, core = hdbscan_tree(X, 10, alpha=alpha, core_method="learned")
+        st = _lib.learned_stats()
+    finally:
+        _lib.set_pruning("auto")
+    Zo, co = oracle.hdbscan_tree(X, 10, alpha=alpha)
+    np.testing.assert_array_equal(core, co)
+    np.testing.assert_array_equal(Z, Zo)
+    assert st["settled_rows"] == 0 and st["offered_rows"] == 0
