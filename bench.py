@@ -285,6 +285,9 @@ def main():
                         help="leave out the nested HDBSCAN (1M, d=8) block of the default run")
     parser.add_argument("--host-steps", type=int, default=2,
                         help="trees timed from host memory (cuhc_linkage, PCIe included)")
+    parser.add_argument("--print-maps", action="store_true",
+                        help="print the memory-map lines of libcuhc.so and the HIP runtime "
+                             "after loading (to attribute a crash address)")
     parser.add_argument("--calibrate-copy-mib", type=int, default=0,
                         help="only run a device-to-device copy of this size (counter scaling)")
     args = parser.parse_args()
@@ -295,6 +298,12 @@ def main():
 
     import cuhc_amd  # noqa: F401
     from cuhc_amd import _lib
+
+    if args.print_maps:
+        with open("/proc/self/maps") as maps:
+            for line in maps:
+                if any(k in line for k in ("libcuhc", "libamdhip64", "libhsa-runtime")):
+                    log(f"maps: {line.rstrip()}")
 
     if args.calibrate_copy_mib > 0:
         calibrate_copy(args.calibrate_copy_mib)

@@ -7,6 +7,7 @@ interface (python/cuhc/_distances.py); the checks are written here.
 import numpy as np
 
 from . import _lib
+from ._device import DeviceMatrix, device_matrix, feature_ranges
 
 # Metric names (scikit-learn's, with aliases) -> the C ABI metric id.
 METRIC_IDS = {
@@ -56,13 +57,34 @@ def as_points(array, name):
     return points
 
 
+def input_points(array, name):
+    """(host points, None) for host data (as_points), or (None, DeviceMatrix)
+    for a float32 matrix already in GPU memory (CUDA array interface or a ROCm
+    DLPack tensor), which is read in place."""
+    dm = device_matrix(array, name)
+    if dm is not None:
+        return None, dm
+    return as_points(array, name), None
+
+
+def ranges_of(X):
+    """Per-feature (min, max) of host points or of a DeviceMatrix (on the GPU)."""
+    if isinstance(X, DeviceMatrix):
+        return feature_ranges(_lib.lib, X)
+    return X.min(axis=0), X.max(axis=0)
+
+
 def check_distance_range(X, metric, p=2.0, Y=None, scale=1.0):
     """Raises ValueError when some FP32 distance (times `scale`: Ward's cluster
     sizes, HDBSCAN's 1 / alpha) could overflow float32. The bound is exact per
-    feature, so data far from the origin passes as long as its spread fits."""
-    lo, hi = X.min(axis=0), X.max(axis=0)
+    feature, so data far from the origin passes as long as its spread fits.
+    X and Y: host points or DeviceMatrix views."""
+    lo, hi = ranges_of(X)
     if Y is not None:
-        lo, hi = np.minimum(lo, Y.min(axis=0)), np.maximum(hi, Y.max(axis=0))
+        ylo, yhi = ranges_of(Y)
+        lo, hi = np.minimum(lo, ylo), np.maximum(hi, yhi)
+    if not (np.isfinite(lo).all() and np.isfinite(hi).all()):
+        raise ValueError("X contains NaN or infinity")
     lo, hi = lo.astype(np.float64), hi.astype(np.float64)
     with np.errstate(over="ignore", invalid="ignore"):
         bound = float(_ACCUMULATED_BOUND[METRIC_IDS[metric]](
@@ -87,7 +109,9 @@ def pairwise_distances(X, Y=None, metric="euclidean", p=2.0):
            [5., 0.]], dtype=float32)
     """
     validate_metric(metric, p)
-    A = as_points(X, "X")
+    A, dA = input_points(X, "X")
+    if dA is not None:
+        return _pairwise_on_device(dA, Y, metric, p)
     B = None if Y is None else as_points(Y, "Y")
     if B is not None and B.shape[1] != A.shape[1]:
         raise ValueError(
@@ -100,3 +124,30 @@ def pairwise_distances(X, Y=None, metric="euclidean", p=2.0):
         _lib.float_ptr(A), len(A), None if B is None else _lib.float_ptr(B), len(cols),
         A.shape[1], METRIC_IDS[metric], float(p), _lib.float_ptr(out)))
     return out
+
+
+def _pairwise_on_device(dA, Y, metric, p):
+    """pairwise_distances for X in GPU memory: Y must be in GPU memory too (or
+    None), and the matrix stays there (a DeviceArray)."""
+    with dA:
+        dB = None
+        if Y is not None:
+            dB = device_matrix(Y, "Y")
+            if dB is None:
+                raise ValueError("Y must be in GPU memory when X is")
+        try:
+            if dB is not None and dB.shape[1] != dA.shape[1]:
+                raise ValueError(f"X and Y must have the same number of features, got "
+                                 f"{dA.shape[1]} and {dB.shape[1]}")
+            check_distance_range(dA, metric, p, dB)
+            cols = dA if dB is None else dB
+            out = _lib.DeviceArray.empty((dA.shape[0], cols.shape[0]), np.float32)
+            import ctypes
+            _lib.check(_lib.lib.cuhc_pairwise_distances_device(
+                ctypes.c_void_p(dA.ptr), dA.shape[0],
+                None if dB is None else ctypes.c_void_p(dB.ptr), cols.shape[0], dA.shape[1],
+                METRIC_IDS[metric], float(p), out.ptr))
+            return out
+        finally:
+            if dB is not None:
+                dB.close()

@@ -10,6 +10,7 @@ import numpy as np
 from sklearn.base import BaseEstimator, ClusterMixin
 
 from ._distances import as_points
+from ._device import gpu_shape
 from ._linkage import LINKAGE_IDS, cut_tree, linkage, resolve_algorithm, validate_linkage
 
 
@@ -56,8 +57,9 @@ class AgglomerativeClustering(ClusterMixin, BaseEstimator):
             raise ValueError(message)
         validate_linkage(self.linkage, self.metric)
         resolve_algorithm(self.algorithm, self.linkage)
-        points = as_points(X, "X")
-        n = len(points)
+        shape = gpu_shape(X)
+        points = X if shape is not None else as_points(X, "X")  # GPU data is read in place
+        n = (shape or points.shape)[0]
         if self.n_clusters is not None and self.n_clusters > n:
             raise ValueError(f"Cannot extract more clusters ({self.n_clusters}) than samples ({n})")
         Z = linkage(points, method=self.linkage, metric=self.metric, algorithm=self.algorithm)
@@ -72,6 +74,6 @@ class AgglomerativeClustering(ClusterMixin, BaseEstimator):
         if thresholded or self.compute_distances:
             self.distances_ = heights.copy()
         self.n_leaves_ = n
-        self.n_features_in_ = points.shape[1]
+        self.n_features_in_ = (shape or points.shape)[1]
         self.n_connected_components_ = 1
         return self

@@ -16,7 +16,9 @@ import numpy as np
 from sklearn.base import BaseEstimator, ClusterMixin
 
 from . import _lib
-from ._distances import METRIC_IDS, as_points, check_distance_range, validate_metric
+from ._device import gpu_shape
+from ._distances import (METRIC_IDS, as_points, check_distance_range, input_points,
+                         validate_metric)
 
 NOISE = -1
 CORE_METHOD_IDS = {"auto": 0, "fused": 1, "radix": 2, "learned": 3}
@@ -24,15 +26,28 @@ SELECTION_METHOD_IDS = {"eom": 0, "leaf": 1}
 
 
 def hdbscan_tree(X, min_samples, metric="euclidean", alpha=1.0, p=2.0, core_method="auto"):
-    """(single-linkage tree of mutual reachability (n-1, 4), core distances (n,))."""
-    X = as_points(X, "X")
-    check_distance_range(X, metric, p, scale=1.0 / alpha)
-    n, d = X.shape
-    tree = np.empty((max(n - 1, 0), 4), dtype=np.float64)
-    core = np.empty(n, dtype=np.float32)
-    _lib.check(_lib.lib.cuhc_hdbscan_tree(
-        _lib.float_ptr(X), n, d, METRIC_IDS[metric], float(p), int(min_samples), float(alpha),
-        CORE_METHOD_IDS[core_method], _lib.double_ptr(tree), _lib.float_ptr(core)))
+    """(single-linkage tree of mutual reachability (n-1, 4), core distances (n,)).
+    X: host points, or a float32 matrix in GPU memory (read in place)."""
+    points, device = input_points(X, "X")
+    source = points if device is None else device
+    try:
+        check_distance_range(source, metric, p, scale=1.0 / alpha)
+        n, d = source.shape
+        tree = np.empty((max(n - 1, 0), 4), dtype=np.float64)
+        core = np.empty(n, dtype=np.float32)
+        args = (n, d, METRIC_IDS[metric], float(p), int(min_samples), float(alpha),
+                CORE_METHOD_IDS[core_method], _lib.double_ptr(tree), _lib.float_ptr(core))
+        if device is None:
+            status = _lib.lib.cuhc_hdbscan_tree(_lib.float_ptr(points), *args)
+        else:
+            import ctypes
+            status = _lib.lib.cuhc_hdbscan_tree_device(ctypes.c_void_p(device.ptr), *args)
+            if status == _lib.NON_FINITE:
+                raise ValueError("X contains NaN or infinity")
+        _lib.check(status)
+    finally:
+        if device is not None:
+            device.close()
     return tree, core
 
 
@@ -346,15 +361,19 @@ class HDBSCAN(ClusterMixin, BaseEstimator):
 
     def fit(self, X, y=None):
         self._check_parameters()
-        X = as_points(X, "X")
+        shape = gpu_shape(X)
+        if shape is None:  # host data: converted and checked once
+            X = as_points(X, "X")
+            shape = X.shape
         k = self.min_samples if self.min_samples is not None else self.min_cluster_size
-        _require(k <= len(X),
-                 f"min_samples ({k}) must be at most the number of samples in X ({len(X)})")
+        _require(k <= shape[0],
+                 f"min_samples ({k}) must be at most the number of samples in X ({shape[0]})")
         Z, core = hdbscan_tree(X, k, metric=self.metric, alpha=self.alpha)
         self.labels_, self.probabilities_ = tree_to_labels(
             Z, self.min_cluster_size, self.cluster_selection_method, self.allow_single_cluster,
             self.cluster_selection_epsilon, self.max_cluster_size)
         self._single_linkage_tree_ = Z
         self.core_distances_ = core
-        self.n_features_in_ = X.shape[1]
+        self.n_features_in_ = shape[1]
         return self
+

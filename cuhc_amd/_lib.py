@@ -60,6 +60,9 @@ _declare("cuhc_radix_sort_pairs_u32", _int, ctypes.c_void_p, ctypes.c_void_p, ct
          ctypes.c_void_p, _int, _int)
 _declare("cuhc_radix_sort_pairs_u64", _int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
          ctypes.c_void_p, _int, _int)
+_declare("cuhc_pairwise_distances_device", _int, ctypes.c_void_p, _int, ctypes.c_void_p, _int,
+         _int, _int, _float, ctypes.c_void_p)
+_declare("cuhc_feature_ranges_device", _int, ctypes.c_void_p, _int, _int, _f32p, _f32p)
 _declare("cuhc_hdbscan_tree_device", _int, ctypes.c_void_p, _int, _int, _int, _float, _int,
          _float, _int, _f64p, _f32p)
 _declare("cuhc_device_alloc", _int, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_void_p))
@@ -84,6 +87,8 @@ PRECISION_MODES = {"auto": 0, "fp32": 1, "mixed": 2}
 PHASES = ["distance_tiles", "mfma_candidates", "recheck", "core", "contract", "dense", "rnn",
           "mlp", "dense_select", "dense_update", "dense_rescan", "dense_barrier"]
 
+# The library's status for NaN or infinite input (reference src/mst.cuh:80-81).
+NON_FINITE = -6
 # hipErrorOutOfMemory, the status the dense path returns when its matrix does not fit.
 HIP_ERROR_OUT_OF_MEMORY = 2
 
@@ -146,17 +151,43 @@ def read_timings():
 
 
 class DeviceArray:
-    """A host array copied into HBM with the library's allocation and copy entry
-    points; .ptr is the device address the *_device entry points take."""
+    """A C-contiguous array in HBM, allocated with the library's allocation
+    entry point: a copy of a host array, or uninitialised (DeviceArray.empty).
+    .ptr is the device address the *_device entry points take; the array
+    exports the CUDA array interface (version 3), so cuhc_amd's functions
+    read it in place, and .to_host() copies it back."""
 
-    def __init__(self, host):
+    def __init__(self, host=None, *, shape=None, dtype=None):
         import numpy as np
-        self.host = np.ascontiguousarray(host)
-        self.shape, self.dtype = self.host.shape, self.host.dtype
+        if host is not None:
+            self.host = np.ascontiguousarray(host)
+            self.shape, self.dtype = self.host.shape, self.host.dtype
+            nbytes = self.host.nbytes
+        else:
+            self.host = None
+            self.shape, self.dtype = tuple(shape), np.dtype(dtype)
+            nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.nbytes = nbytes
         self.ptr = ctypes.c_void_p()
-        check(lib.cuhc_device_alloc(self.host.nbytes, ctypes.byref(self.ptr)))
-        check(lib.cuhc_copy_to_device(self.ptr, self.host.ctypes.data_as(ctypes.c_void_p),
-                                      self.host.nbytes))
+        check(lib.cuhc_device_alloc(max(nbytes, 1), ctypes.byref(self.ptr)))
+        if host is not None:
+            check(lib.cuhc_copy_to_device(self.ptr, self.host.ctypes.data_as(ctypes.c_void_p),
+                                          nbytes))
+
+    @classmethod
+    def empty(cls, shape, dtype):
+        return cls(shape=shape, dtype=dtype)
+
+    @property
+    def __cuda_array_interface__(self):
+        return {"shape": tuple(self.shape), "typestr": self.dtype.str, "strides": None,
+                "data": (int(self.ptr.value or 0), False), "version": 3}
+
+    def to_host(self):
+        import numpy as np
+        out = np.empty(self.shape, self.dtype)
+        check(lib.cuhc_copy_to_host(out.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes))
+        return out
 
     def free(self):
         if self.ptr:

@@ -9,7 +9,8 @@ from scipy.sparse import coo_matrix
 from scipy.sparse.csgraph import connected_components
 
 from . import _lib
-from ._distances import METRIC_IDS, as_points, check_distance_range, validate_metric
+from ._distances import (METRIC_IDS, as_points, check_distance_range, input_points,
+                         validate_metric)
 
 LINKAGE_IDS = {"single": 0, "complete": 1, "average": 2, "ward": 3}
 ALGORITHM_IDS = {"dense": 0, "mst": 1, "nn_chain": 2, "rnn": 3}
@@ -64,15 +65,28 @@ def linkage(X, method="single", metric="euclidean", p=2.0, *, algorithm="auto"):
     validate_linkage(method, metric)
     chosen = resolve_algorithm(algorithm, method)
     validate_metric(metric, p)
-    points = as_points(X, "X")
-    n, d = points.shape
-    # Ward's squared heights reach 2 a b / (a + b) <= n times a squared point distance.
-    check_distance_range(points, metric, p, scale=float(n) if method == "ward" else 1.0)
-    Z = np.empty((max(n - 1, 0), 4), dtype=np.float64)
-    if n < 2:
-        return Z
-    status = _lib.lib.cuhc_linkage(_lib.float_ptr(points), n, d, METRIC_IDS[metric], float(p),
-                                   LINKAGE_IDS[method], ALGORITHM_IDS[chosen], _lib.double_ptr(Z))
+    points, device = input_points(X, "X")
+    source = points if device is None else device
+    try:
+        n, d = source.shape
+        # Ward's squared heights reach 2 a b / (a + b) <= n times a squared point distance.
+        check_distance_range(source, metric, p, scale=float(n) if method == "ward" else 1.0)
+        Z = np.empty((max(n - 1, 0), 4), dtype=np.float64)
+        if n < 2:
+            return Z
+        args = (n, d, METRIC_IDS[metric], float(p), LINKAGE_IDS[method], ALGORITHM_IDS[chosen],
+                _lib.double_ptr(Z))
+        if device is None:
+            status = _lib.lib.cuhc_linkage(_lib.float_ptr(points), *args)
+        else:
+            # Points read in place from GPU memory (no copy to the host).
+            import ctypes
+            status = _lib.lib.cuhc_linkage_device(ctypes.c_void_p(device.ptr), *args)
+            if status == _lib.NON_FINITE:
+                raise ValueError("X contains NaN or infinity")
+    finally:
+        if device is not None:
+            device.close()
     if chosen == "dense" and status == _lib.HIP_ERROR_OUT_OF_MEMORY:
         raise MemoryError(_dense_memory_message(n, method))
     _lib.check(status)

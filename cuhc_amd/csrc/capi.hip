@@ -78,12 +78,31 @@ int linkageImpl(const float *X, bool onDevice, int n, int d, int metricId, float
     gPruneStats.tilesTotal += stats.tilesTotal;
     return s.code;
   }
+  if (algorithmId == kAlgRnn) {
+    // Ward is euclidean only; its passes work on squared distances
+    // (reference src/capi.cu:210-222: pruned when the points allow it).
+    const MetricSpec sq{kSqEuclidean, 2.0f};
+    bool prune = shouldPrune(gPruneMode.load(), kSqEuclidean, n, d);
+    DevicePoints pts;
+    s(onDevice ? pts.adopt(X, n, d, prune) : pts.upload(X, n, d, prune));
+    if (s.ok())
+      s(decidePruning(pts, n, d, sq, prune));
+    PassStats stats;
+    RnnOptions opt;
+    opt.prune = prune && pts.ordered();
+    opt.orig = pts.ordered() ? pts.orig.get() : nullptr;
+    opt.stats = &stats;
+    if (s.ok())
+      s(rnnWard(pts.data(), n, d, opt, tree));
+    std::lock_guard<std::mutex> lock(gStatsMutex);
+    gPruneStats.tilesComputed += stats.tilesComputed;
+    gPruneStats.tilesTotal += stats.tilesTotal;
+    return s.code;
+  }
   DevicePoints pts;
   s(onDevice ? pts.adopt(X, n, d, false) : pts.upload(X, n, d, false));
   if (!s.ok())
     return s.code;
-  if (algorithmId == kAlgRnn)
-    return rnnWard(pts.data(), n, d, RnnOptions{}, tree);
   if (algorithmId == kAlgNnChain)
     return nnChainWard(pts.data(), n, d, tree);
   return denseLinkage(pts.data(), n, d, m, linkageId, tree);
@@ -153,6 +172,30 @@ int cuhc_pairwise_distances(const float *A, int nA, const float *B, int nB, int 
                      dOut));
   if (s.ok())
     s(download(distances, dOut, (size_t)nA * nB * sizeof(float)));
+  return s.code;
+}
+
+int cuhc_pairwise_distances_device(const float *A, int nA, const float *B, int nB, int d,
+                                   int metricId, float metricParam, float *distances) {
+  if (nA <= 0 || nB <= 0 || d <= 0)
+    return kErrInvalidShape;
+  if (!validMetric(metricId))
+    return kErrInvalidMetric;
+  Status s;
+  s((int)hipDeviceSynchronize()); // the caller's writes may come from any stream
+  if (s.ok())
+    s(pairwiseMatrix(A, nA, B ? B : A, nB, d, MetricSpec{metricId, metricParam}, distances));
+  s(sync());
+  return s.code;
+}
+
+int cuhc_feature_ranges_device(const float *X, int n, int d, float *low, float *high) {
+  if (n <= 0 || d <= 0)
+    return kErrInvalidShape;
+  Status s;
+  s((int)hipDeviceSynchronize());
+  if (s.ok())
+    s(featureRanges(X, n, d, low, high));
   return s.code;
 }
 

@@ -429,7 +429,7 @@ template <class Weight> struct ColumnStage {
       const int cl = min(col, n - 1);
       v.comp = comp[cl];
       v.val = weight.value(cl);
-      v.idx = (unsigned)orig[cl];
+      v.idx = orig ? (unsigned)orig[cl] : (unsigned)cl; // null: keys carry positions
     }
     return v;
   }

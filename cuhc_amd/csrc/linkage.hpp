@@ -29,6 +29,14 @@ struct RnnOptions {
   float minProgress = 0.01f;
   int denseBelow = 0;
   bool allowDense = true;
+  // orig (device, nullable): position -> original index of reordered points;
+  // merges are recorded with original indices. prune (points in k-d order):
+  // the nearest-neighbour passes walk the cluster tiles with box pruning
+  // until a pass computes more than half of the tile pairs. stats (nullable)
+  // receives the passes' tile counts.
+  const int *orig = nullptr;
+  bool prune = false;
+  PassStats *stats = nullptr;
 };
 
 // Ward through rounds of reciprocal nearest neighbours (FP64 centroids and

@@ -66,8 +66,9 @@ struct PassStats {
 };
 
 // Pruned pass for points in spatial order: same contract as
-// nearestOtherSymmetric, except that keys carry orig[j] (device array), so ties
-// break by original index exactly as in an unpruned run. compBest (nullable,
+// nearestOtherSymmetric, except that keys carry orig[j] (device array; null:
+// positions, as for rnn cluster slots), so ties break by original index
+// exactly as in an unpruned run. compBest (nullable,
 // one u32 per component, seedComponentBest) holds the weight bits of the best
 // edge known for each component; rows stop searching above it, so a row's
 // result is exact only when it holds its component's smallest edge, which is

@@ -291,6 +291,67 @@ int checkFinite(const float *X, size_t count) {
   return bad ? kErrNonFinite : 0;
 }
 
+namespace {
+
+// Order-preserving 32-bit image of a float (any sign) for integer atomics.
+HD unsigned int rangeBits(float f) {
+  const unsigned int b = bitsOf(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+HD float rangeFloat(unsigned int o) {
+  return floatOf((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+constexpr int kRangeRows = 1024;
+
+// Workgroup g covers rows [g * 1024, ...); thread t features t, t + 256, ...
+// (consecutive threads read consecutive floats of a row).
+__global__ void __launch_bounds__(256)
+    featureRangesKernel(const float *__restrict__ X, int n, int d, unsigned int *lowHigh) {
+  const int r0 = blockIdx.x * kRangeRows, r1 = min(n, r0 + kRangeRows);
+  for (int f = threadIdx.x; f < d; f += blockDim.x) {
+    float lo = INFINITY, hi = -INFINITY;
+    for (int r = r0; r < r1; ++r) {
+      const float v = X[(size_t)r * d + f];
+      lo = fminf(lo, v);
+      hi = fmaxf(hi, v);
+    }
+    atomicMin(&lowHigh[f], rangeBits(lo));
+    atomicMax(&lowHigh[d + f], rangeBits(hi));
+  }
+}
+
+__global__ void rangeInitKernel(unsigned int *lowHigh, int d) {
+  for (int f = threadIdx.x + blockIdx.x * blockDim.x; f < d; f += gridDim.x * blockDim.x) {
+    lowHigh[f] = 0xffffffffu;
+    lowHigh[d + f] = 0u;
+  }
+}
+
+} // namespace
+
+int featureRanges(const float *X, int n, int d, float *low, float *high) {
+  DevArray<unsigned int> lowHigh;
+  Status s;
+  s(lowHigh.alloc((size_t)2 * d));
+  if (!s.ok())
+    return s.code;
+  hipLaunchKernelGGL(rangeInitKernel, dim3((d + 255) / 256), dim3(256), 0, stream(),
+                     lowHigh.get(), d);
+  hipLaunchKernelGGL(featureRangesKernel, dim3((n + kRangeRows - 1) / kRangeRows), dim3(256), 0,
+                     stream(), X, n, d, lowHigh.get());
+  s((int)hipGetLastError());
+  std::vector<unsigned int> h((size_t)2 * d);
+  s(download(h.data(), lowHigh, h.size() * sizeof(unsigned int)));
+  if (!s.ok())
+    return s.code;
+  for (int f = 0; f < d; ++f) {
+    low[f] = rangeFloat(h[f]);
+    high[f] = rangeFloat(h[d + f]);
+  }
+  return 0;
+}
+
 void releaseCachedBlocks() {
   DeviceCache &cache = deviceCache();
   std::lock_guard<std::mutex> lock(cache.mutex);

@@ -90,6 +90,10 @@ int downloadStaged(const void *src, size_t bytes, const std::function<void(const
 int fill(void *dst, int byte, size_t bytes);
 int sync();
 
+// Per-feature minimum and maximum of the n x d points at X (device) into the
+// host arrays low[d], high[d] (NaN values are ignored).
+int featureRanges(const float *X, int n, int d, float *low, float *high);
+
 // 0 when all count floats at X (device) are finite, kErrNonFinite (-6) when
 // one is NaN or infinite, or a HIP error. One read of the array.
 int checkFinite(const float *X, size_t count);

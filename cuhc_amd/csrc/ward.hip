@@ -180,7 +180,10 @@ __global__ void rnnMergeKernel(Clusters cur, Clusters next, const unsigned long 
   }
 }
 
-__global__ void initClustersKernel(const float *X, int n, int d, Clusters c, int *comp) {
+// Every point its own cluster; rep = its original index (orig, null: the
+// position), comp = its slot.
+__global__ void initClustersKernel(const float *X, int n, int d, Clusters c, int *comp,
+                                   const int *orig) {
   const size_t total = (size_t)n * d;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
        e += (size_t)gridDim.x * blockDim.x) {
@@ -189,7 +192,7 @@ __global__ void initClustersKernel(const float *X, int n, int d, Clusters c, int
   }
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     c.size[i] = 1;
-    c.rep[i] = i;
+    c.rep[i] = orig ? orig[i] : i;
     comp[i] = i;
   }
 }
@@ -509,7 +512,7 @@ int nnChainWard(const float *X, int n, int d, double *tree) {
   s(comp.alloc(n));
   Clusters cl{c, c32, size, rep};
   hipLaunchKernelGGL(initClustersKernel, dim3(1024), dim3(256), 0, stream(), X, n, d, cl,
-                     comp.get());
+                     comp.get(), nullptr);
   std::vector<Edge> merges;
   if (s.ok())
     s(nnChainMerges(c, nullptr, n, d, merges));
@@ -550,12 +553,18 @@ int rnnWard(const float *X, int n, int d, const RnnOptions &opt, double *tree) {
     return s.code;
   Clusters cur{c64a, c32a, sizeA, repA}, next{c64b, c32b, sizeB, repB};
   hipLaunchKernelGGL(initClustersKernel, dim3(1024), dim3(256), 0, stream(), X, n, d, cur,
-                     comp.get());
+                     comp.get(), opt.orig);
   s(fill(numMerges, 0, sizeof(int)));
 
   std::vector<int> roundEnd;
   int m = n, merged = 0;
   enum { kNone, kDense, kChain } tail = kNone;
+  // Pruned passes (points in k-d order): merged clusters take the lower
+  // slot's place in the compacted order, so slots stay spatially coherent and
+  // the cluster tiles keep tight boxes. Every cluster needs its exact
+  // nearest, so no component bound is used; pruning stops for good once a
+  // pass computes more than half of the tile pairs.
+  bool pruneThisPass = opt.prune;
   while (s.ok() && m > 1) {
     if (opt.allowDense && m <= opt.denseBelow && wardMatrixFits(m)) {
       tail = kDense;
@@ -565,7 +574,15 @@ int rnnWard(const float *X, int n, int d, const RnnOptions &opt, double *tree) {
     WeightSpec w;
     w.kind = kWeightWard;
     w.size = cur.size;
-    s(nearestOtherSymmetric(cur.c32, m, d, MetricSpec{kSqEuclidean, 2.0f}, w, comp, nullptr, nearest));
+    if (pruneThisPass) {
+      double share = 1.0;
+      s(nearestOtherWalk(cur.c32, m, d, MetricSpec{kSqEuclidean, 2.0f}, w, comp, nullptr,
+                         nullptr, nearest, &share, opt.stats));
+      pruneThisPass = share <= kPruneGiveUpShare;
+    } else {
+      s(nearestOtherSymmetric(cur.c32, m, d, MetricSpec{kSqEuclidean, 2.0f}, w, comp, nullptr,
+                              nearest));
+    }
     {
       PhaseTimer timer(kPhaseRnn);
       const int blocks = (m + kScanBlock - 1) / kScanBlock;

@@ -133,6 +133,17 @@ int cuhc_radix_sort_pairs_u64(const unsigned long long *keysIn, unsigned long lo
                               const unsigned int *valuesIn, unsigned int *valuesOut, int count,
                               int bits);
 
+/* cuhc_pairwise_distances with every array in device memory (A, B nullable
+ * as in the host form, out nA x nB floats): no PCIe copy. The device is
+ * synchronised before X is read. No counterpart in the reference. */
+int cuhc_pairwise_distances_device(const float *A, int nA, const float *B, int nB, int d,
+                                   int metricId, float metricParam, float *out);
+
+/* Per-feature minimum and maximum of device points (n x d floats) into the
+ * host arrays low[d] and high[d]: the overflow check of the Python surface
+ * (the float32 distance bound) for inputs already in HBM. */
+int cuhc_feature_ranges_device(const float *X, int n, int d, float *low, float *high);
+
 /* cuhc_hdbscan_tree with X in device memory; tree and core are host memory. */
 int cuhc_hdbscan_tree_device(const float *deviceX, int n, int d, int metricId, float metricParam,
                              int minSamples, float alpha, int coreMethod, double *tree,

@@ -153,3 +153,27 @@ def test_agglomerative_matches_sklearn(linkage):
     assert_same_partition(ours.labels_, ref.labels_)
     np.testing.assert_array_equal(np.sort(ours.children_, 1), np.sort(ref.children_, 1))
     np.testing.assert_allclose(ours.distances_, ref.distances_, rtol=1e-4, atol=1e-5)
+
+
+@pytest.mark.parametrize("n_features", [2, 8, 20])
+def test_rnn_pruned_equals_unpruned(n_features):
+    # Reference tests/test_spatial_pruning.py:55-59: the pruned rnn passes
+    # (points in k-d order, cluster tiles walked with box bounds) give the
+    # unpruned dendrogram, and both give scipy's.
+    X = blobs(n_samples=6000, n_features=n_features, centers=30, seed=10 + n_features)
+    trees, stats = {}, {}
+    try:
+        for mode in ["always", "never"]:
+            from cuhc_amd import _lib
+            _lib.set_pruning(mode)
+            _lib.prune_stats()
+            trees[mode] = cuhc.linkage(X, method="ward", algorithm="rnn")
+            stats[mode] = _lib.prune_stats()
+    finally:
+        _lib.set_pruning("auto")
+    assert_same_dendrogram(trees["always"], trees["never"], rtol=1e-5, atol=1e-6)
+    assert_same_dendrogram(trees["always"], scipy_ward(X), rtol=1e-5, atol=1e-6)
+    computed, total = stats["always"]
+    print(n_features, stats)
+    assert 0 < computed < total  # some tile pairs were skipped
+    assert stats["never"] == (0, 0)
