@@ -604,12 +604,17 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
         runTile(std::true_type{});
       else
         runTile(std::false_type{});
-      // Tile end: both halves of each row agree on the row's kTopK smallest
-      // half-block minima and theta. The merged list stays with the lane of
-      // half 0 and half 1 starts empty, so no minimum is counted twice; after
-      // the dry run both start empty (its tile is visited again).
+      // Tile end (every second tile, the dry run and the last tile): both
+      // halves of each row agree on the row's kTopK smallest half-block minima
+      // and theta. The merged list stays with the lane of half 0 and half 1
+      // starts empty, so no minimum is counted twice; after the dry run both
+      // start empty (its tile is visited again). Between merges theta stays
+      // that of the last merge, only looser: more blocks are recorded, none
+      // is missed. (Every fourth tile records so many more that lists
+      // overflow.)
+      const bool mergeNow = dry || (ii & 1) || ii + 1 == iters;
 #pragma unroll
-      for (int rb = 0; rb < 4; ++rb) {
+      for (int rb = 0; rb < 4 && mergeNow; ++rb) {
         float c[kTopK];
 #pragma unroll
         for (int q = 0; q < kTopK; ++q)
