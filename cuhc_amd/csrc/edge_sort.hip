@@ -13,10 +13,7 @@ namespace cuhc {
 
 namespace {
 
-struct PackedEdge {
-  int u, v;
-  float w;
-};
+using PackedEdge = Edge32;
 
 __global__ void edgeKeysKernel(const int *__restrict__ u, const float *__restrict__ w, int m,
                                int uBits, unsigned long long *__restrict__ keys,
@@ -47,9 +44,9 @@ int treeFromDeviceEdges(const DeviceEdges &edges, int n, double *tree) {
   const int m = edges.count;
   if (m <= 0)
     return 0;
-  std::vector<Edge> host(m);
   Status s;
   if (m < 65536) {
+    std::vector<Edge> host(m);
     std::vector<int> hu(m), hv(m);
     std::vector<float> hw(m);
     s(download(hu.data(), edges.u, m * sizeof(int)));
@@ -60,50 +57,49 @@ int treeFromDeviceEdges(const DeviceEdges &edges, int n, double *tree) {
     for (int i = 0; i < m; ++i)
       host[i] = Edge{hu[i], hv[i], (double)hw[i]};
     sortEdges(host);
-  } else {
-    int uBits = 1;
-    while (uBits < 31 && (n - 1) >> uBits)
-      ++uBits;
-    DevArray<unsigned long long> keys, keysOut;
-    DevArray<unsigned int> idx, idxOut;
-    DevArray<PackedEdge> packed;
-    DevArray<unsigned char> temp;
-    s(keys.alloc(m));
-    s(keysOut.alloc(m));
-    s(idx.alloc(m));
-    s(idxOut.alloc(m));
-    s(packed.alloc(m));
-    s(temp.alloc(sortTempBytes64(m)));
-    if (!s.ok())
-      return s.code;
-    const dim3 grid((m + 255) / 256);
-    hipLaunchKernelGGL(edgeKeysKernel, grid, dim3(256), 0, stream(), edges.u.get(),
-                       edges.w.get(), m, uBits, keys.get(), idx.get());
-    s((int)hipGetLastError());
-    s(radixSortPairs(keys.get(), keysOut.get(), idx.get(), idxOut.get(), m, 32 + uBits,
-                     temp.get()));
-    hipLaunchKernelGGL(edgeGatherKernel, grid, dim3(256), 0, stream(), edges.u.get(),
-                       edges.v.get(), edges.w.get(), idxOut.get(), m, packed.get());
-    s((int)hipGetLastError());
-    s(downloadStaged(packed, m * sizeof(PackedEdge), [&](const void *p) {
-      const PackedEdge *hp = static_cast<const PackedEdge *>(p);
-      for (int i = 0; i < m; ++i)
-        host[i] = Edge{hp[i].u, hp[i].v, (double)hp[i].w};
-    }));
-    if (!s.ok())
-      return s.code;
-    // Runs of equal (weight, u): order by v.
+    treeFromOrderedMerges(host, n, tree);
+    return 0;
+  }
+  int uBits = 1;
+  while (uBits < 31 && (n - 1) >> uBits)
+    ++uBits;
+  DevArray<unsigned long long> keys, keysOut;
+  DevArray<unsigned int> idx, idxOut;
+  DevArray<PackedEdge> packed;
+  DevArray<unsigned char> temp;
+  s(keys.alloc(m));
+  s(keysOut.alloc(m));
+  s(idx.alloc(m));
+  s(idxOut.alloc(m));
+  s(packed.alloc(m));
+  s(temp.alloc(sortTempBytes64(m)));
+  if (!s.ok())
+    return s.code;
+  const dim3 grid((m + 255) / 256);
+  hipLaunchKernelGGL(edgeKeysKernel, grid, dim3(256), 0, stream(), edges.u.get(),
+                     edges.w.get(), m, uBits, keys.get(), idx.get());
+  s((int)hipGetLastError());
+  s(radixSortPairs(keys.get(), keysOut.get(), idx.get(), idxOut.get(), m, 32 + uBits,
+                   temp.get()));
+  hipLaunchKernelGGL(edgeGatherKernel, grid, dim3(256), 0, stream(), edges.u.get(),
+                     edges.v.get(), edges.w.get(), idxOut.get(), m, packed.get());
+  s((int)hipGetLastError());
+  // The tree is written straight from the staged edges: runs of equal
+  // (weight, u) are ordered by v in place first.
+  s(downloadStaged(packed, m * sizeof(PackedEdge), [&](const void *p) {
+    PackedEdge *hp = static_cast<PackedEdge *>(const_cast<void *>(p));
     for (int i = 0; i < m;) {
       int j = i + 1;
-      while (j < m && host[j].w == host[i].w && host[j].u == host[i].u)
+      while (j < m && hp[j].w == hp[i].w && hp[j].u == hp[i].u)
         ++j;
       if (j - i > 1)
-        std::sort(host.begin() + i, host.begin() + j, edgeLess);
+        std::sort(hp + i, hp + j,
+                  [](const PackedEdge &x, const PackedEdge &y) { return x.v < y.v; });
       i = j;
     }
-  }
-  treeFromOrderedMerges(host, n, tree);
-  return 0;
+    treeFromOrderedEdges(hp, (size_t)m, n, tree);
+  }));
+  return s.code;
 }
 
 } // namespace cuhc

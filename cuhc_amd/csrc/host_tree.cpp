@@ -16,12 +16,20 @@ struct Node {
   double height;
 };
 
+// The records live in a per-thread buffer kept between calls: a fresh 24 MB
+// array at 10^6 points costs its page faults on every tree.
 struct Clusters {
-  std::vector<Node> node;
-  explicit Clusters(int n) : node(n) {
+  Node *node;
+  explicit Clusters(int n) {
+    static thread_local std::vector<Node> buffer;
+    if (buffer.size() < (size_t)n)
+      buffer.resize(n);
+    node = buffer.data();
     for (int i = 0; i < n; ++i)
       node[i] = {i, 1, i, 0.0};
   }
+  Clusters(const Clusters &) = delete;
+  Clusters &operator=(const Clusters &) = delete;
   int find(int x) {
     int r = x;
     while (node[r].link != r)
@@ -63,13 +71,14 @@ bool mergesToTreeOrder(std::vector<Edge> &merges, int n) {
   return true;
 }
 
-void treeFromOrderedMerges(const std::vector<Edge> &merges, int n, double *tree) {
+namespace {
+
+template <class E> void assembleTree(const E *merges, size_t m, int n, double *tree) {
   Clusters sets(n);
-  std::vector<Node> &node = sets.node;
+  Node *node = sets.node;
   // The finds start at random points: the records of the endpoints 16 merges
   // ahead, and their links 8 merges ahead, are prefetched.
   constexpr size_t kAhead = 16;
-  const size_t m = merges.size();
   for (size_t s = 0; s < m; ++s) {
     if (s + kAhead < m) {
       __builtin_prefetch(&node[merges[s + kAhead].u], 1);
@@ -84,10 +93,20 @@ void treeFromOrderedMerges(const std::vector<Edge> &merges, int n, double *tree)
     double *row = tree + s * 4;
     row[0] = std::min(a, b);
     row[1] = std::max(a, b);
-    row[2] = merges[s].w;
+    row[2] = (double)merges[s].w;
     row[3] = node[ra].size + node[rb].size;
     node[sets.join(ra, rb)].id = n + (int)s;
   }
+}
+
+} // namespace
+
+void treeFromOrderedMerges(const std::vector<Edge> &merges, int n, double *tree) {
+  assembleTree(merges.data(), merges.size(), n, tree);
+}
+
+void treeFromOrderedEdges(const Edge32 *edges, size_t m, int n, double *tree) {
+  assembleTree(edges, m, n, tree);
 }
 
 void sortEdges(std::vector<Edge> &edges) { std::sort(edges.begin(), edges.end(), edgeLess); }

@@ -3,6 +3,7 @@
 // order follow reference src/mst.cuh:36-48 and 204-235.
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 #include <vector>
 
@@ -35,6 +36,13 @@ bool mergesToTreeOrder(std::vector<Edge> &merges, int n);
 // Scipy linkage rows {min id, max id, height, size} from merges already in
 // tree order; the cluster made by merge s is n + s.
 void treeFromOrderedMerges(const std::vector<Edge> &merges, int n, double *tree);
+
+// The same from FP32-weighted edges (the layout the device sort produces).
+struct Edge32 {
+  int u, v; // u < v
+  float w;
+};
+void treeFromOrderedEdges(const Edge32 *edges, size_t m, int n, double *tree);
 
 // Sorts by edgeLess on the host.
 void sortEdges(std::vector<Edge> &edges);
