@@ -184,12 +184,38 @@ C3_WORKLOAD = "HDBSCAN mcs=min_samples=25, n=1,000,000, d=8, 20 blobs + 10% nois
 CORE_METHODS = {"auto": 0, "fused": 1, "radix": 2, "learned": 3}
 
 
-def c3_block(methods, steps, radix_steps=1):
+def _large_k_block(lib, dev, n, d, k=100):
+    """The same points with min_samples = 100, above the fused top-k's k <= 64:
+    auto hands the core distances to the learned radius; the method without
+    it is the full-row radix select. One tree each, identical trees."""
+    from cuhc_amd import _lib
+    tree = np.empty((n - 1, 4))
+    core = np.empty(n, np.float32)
+    out, results = {"min_samples": k}, []
+    for method in ["auto", "radix"]:
+        _lib.learned_stats()
+        t = timed_steps(lambda: _lib.check(lib.cuhc_hdbscan_tree_device(  # noqa: B023
+            dev.ptr, n, d, 0, 2.0, k, 1.0, CORE_METHODS[method], _lib.double_ptr(tree),
+            _lib.float_ptr(core))), 1)
+        out[method] = {"tree_s": t}
+        if method == "auto":
+            out[method]["learned_stats"] = _lib.learned_stats()
+        results.append((tree.copy(), core.copy()))
+        log(f"c3 k={k} {method}: {t:.3f} s")
+    out["identical"] = bool(np.array_equal(results[0][0], results[1][0]) and
+                            np.array_equal(results[0][1], results[1][1]))
+    out["radix_vs_auto"] = out["radix"]["tree_s"] / out["auto"]["tree_s"]
+    return out
+
+
+def c3_block(methods, steps, radix_steps=1, large_k=True):
     """HDBSCAN trees of the 1M-point 8-d workload from points resident in HBM,
     one entry per core-distance method (auto: fused k-d top-k + pruned
     Boruvka; learned: the radius network's proposals, verified; radix: the
     full-row radix select), with the identity of their trees and core
-    distances and the extraction time."""
+    distances and the extraction time; large_k adds the same points at
+    min_samples = 100 (auto, which then uses the learned radius, against the
+    radix select)."""
     from cuhc_amd import _lib
     from cuhc_amd._hdbscan import tree_to_labels
     lib = _lib.lib
@@ -200,6 +226,8 @@ def c3_block(methods, steps, radix_steps=1):
     core = np.empty(n, np.float32)
     out, trees = {"workload": C3_WORKLOAD}, {}
     try:
+        if large_k:
+            out["k100"] = _large_k_block(lib, dev, n, d)
         for method in methods:
             call = lambda: _lib.check(lib.cuhc_hdbscan_tree_device(  # noqa: E731
                 dev.ptr, n, d, 0, 2.0, k, 1.0, CORE_METHODS[method], _lib.double_ptr(tree),

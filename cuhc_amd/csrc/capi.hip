@@ -49,6 +49,13 @@ int decidePruning(const DevicePoints &pts, int n, int d, MetricSpec m, bool &pru
   return status;
 }
 
+// Points from which auto picks the learned radius for k > 64: below, the
+// radix select's full rows cost less than the network's fixed training.
+constexpr int kLearnedAutoMinPoints = 131072;
+// Largest k auto hands to it: its lists hold at most 4,096 candidates per
+// row, and a proposal needs k of them inside the radius with room to spare.
+constexpr int kLearnedAutoMaxK = 128;
+
 int linkageImpl(const float *X, bool onDevice, int n, int d, int metricId, float metricParam,
                 int linkageId, int algorithmId, double *tree) {
   if (n <= 0 || d <= 0)
@@ -117,10 +124,22 @@ int hdbscanImpl(const float *X, bool onDevice, int n, int d, int metricId, float
   if (!validMetric(metricId))
     return kErrInvalidMetric;
   const MetricSpec m{metricId, metricParam};
+  // auto: the fused top-k up to k = 64 (reference src/hdbscan.cuh:324); above
+  // it the learned radius for box metrics on large inputs (its proposals are
+  // verified exactly, and it computes a small share of the pairs the
+  // full-row radix select does), else the radix select. Every method gives
+  // the same core distances.
+  int method = coreMethod;
+  if (method == kCoreAuto)
+    method = minSamples <= kMaxFusedK ? kCoreFused
+             : (metricHasBoxBound(metricId) && n >= kLearnedAutoMinPoints &&
+                minSamples <= kLearnedAutoMaxK)
+                 ? kCoreLearned
+                 : kCoreRadix;
   bool prune = shouldPrune(gPruneMode.load(), metricId, n, d);
   // The learned radii are predicted from k-d tiles: spatial order pays for it
   // at any size (the results are the same in every order).
-  const bool spatial = prune || (coreMethod == kCoreLearned && n > 1 &&
+  const bool spatial = prune || (method == kCoreLearned && n > 1 &&
                                  gPruneMode.load() != kPruneNever && metricHasBoxBound(metricId));
   DevicePoints pts;
   Status s;
@@ -133,7 +152,7 @@ int hdbscanImpl(const float *X, bool onDevice, int n, int d, int metricId, float
       if (core)
         core[0] = 0.0f;
     } else {
-      s(hdbscanTree(pts, n, d, m, minSamples, alpha, coreMethod, prune, &stats, tree, core));
+      s(hdbscanTree(pts, n, d, m, minSamples, alpha, method, prune, &stats, tree, core));
     }
   }
   std::lock_guard<std::mutex> lock(gStatsMutex);

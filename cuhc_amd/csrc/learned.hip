@@ -5,15 +5,16 @@
 //    log of the k-th smallest distance within its own tile and within its tile
 //    and the two neighbouring tiles, the log of the tile's box diagonal and of
 //    the point's distance to the tile centroid.
-// 2. A training sample of 4,096 evenly spaced points gets its exact core
-//    distances (full rows, radix select); the network (mlp.hip) is trained on
+// 2. A training sample of 64 row tiles (8,192 points) spread evenly over
+//    the k-d order gets its exact core distances (the pruned fused walk of
+//    those tiles; full rows when k > 64); the network (mlp.hip) is trained on
 //    the GPU to predict their log, and the proposal margin is the 99.5%
 //    residual quantile on the sample plus 0.02.
 // 3. Every point gets a radius r_i = exp(prediction + margin). One workgroup
 //    per row tile walks the column tiles and skips those whose box lower
 //    bound (with the pruning slack) exceeds every row's radius; every
 //    distance <= r_i (the point itself as 0) is appended to the row's
-//    candidate list (512 per row).
+//    candidate list (512 slots per row for k <= 64, up to 4,096 above).
 // 4. A wave per row radix-selects the k-th smallest candidate (32 one-bit
 //    passes with 64-bit ballots). The list holds every point within r_i, so
 //    when it has at least k entries (and did not overflow) the k-th smallest
@@ -33,7 +34,17 @@ namespace cuhc {
 
 namespace {
 
-constexpr int kCandPerRow = 512;
+// Candidate slots per row: 512 up to k = 64 (the fused top-k's range), 32 k
+// rounded up to a power of two above (at most 4,096). The radius carries a
+// margin over the predicted core distance, and in d dimensions the points
+// inside grow like the margin's d-th power (at d = 8 a 22% margin holds about
+// five times k points, and the margin grows with the spread of the residuals).
+int candSlotsFor(int k) {
+  int cap = 512;
+  while (k > 64 && cap < 32 * k && cap < 4096)
+    cap *= 2;
+  return cap;
+}
 constexpr int kTrainSamples = 8192;
 std::atomic<int> gTrainSteps{300};
 constexpr float kTrainRate = 0.01f;
@@ -132,7 +143,7 @@ __global__ void __launch_bounds__(256)
     radiusGatherKernel(const float *__restrict__ X, int n, int d, Metric metric,
                        const float *__restrict__ radius, const float *__restrict__ low,
                        const float *__restrict__ high, float *__restrict__ cand,
-                       int *__restrict__ candPos, int *__restrict__ count,
+                       int *__restrict__ candPos, int *__restrict__ count, int cap,
                        unsigned long long *tilesComputed) {
   constexpr int TR = T::TR;
   __shared__ __align__(16) TileSmem<T> sm;
@@ -203,10 +214,10 @@ __global__ void __launch_bounds__(256)
           const float val = col == row ? 0.0f : fmaxf(metric.finish(v[r][c]), 0.0f);
           if (val <= rRow[r]) {
             const int pos = atomicAdd(&cnt[rit], 1);
-            if (pos < kCandPerRow) {
-              cand[(size_t)row * kCandPerRow + pos] = val;
+            if (pos < cap) {
+              cand[(size_t)row * cap + pos] = val;
               if (candPos)
-                candPos[(size_t)row * kCandPerRow + pos] = col;
+                candPos[(size_t)row * cap + pos] = col;
             }
           }
         }
@@ -225,11 +236,12 @@ __global__ void __launch_bounds__(256)
 // One wave per row: the k-th smallest of the row's candidates by a binary
 // radix select over the (non-negative) FP32 bit patterns; rows that cannot be
 // answered from their list go to the fallback list.
+template <int PER> // candidate slots per lane (the row's cap / 64)
 __global__ void __launch_bounds__(256)
     candidateSelectKernel(const float *__restrict__ cand, const int *__restrict__ count, int n,
                           int k, float *__restrict__ core, int *fallback, int *numFallback,
                           unsigned long long *candidates) {
-  constexpr int PER = kCandPerRow / 64;
+  constexpr int kCap = 64 * PER;
   const int lane = threadIdx.x & 63;
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= n)
@@ -237,7 +249,7 @@ __global__ void __launch_bounds__(256)
   const int c = count[row];
   if (lane == 0) // striped over 64 cache lines: one address would serialise a million waves
     atomicAdd(&candidates[(row & 63) * 16], (unsigned long long)c);
-  if (c < k || c > kCandPerRow) {
+  if (c < k || c > kCap) {
     if (lane == 0)
       fallback[atomicAdd(numFallback, 1)] = row;
     return;
@@ -248,7 +260,7 @@ __global__ void __launch_bounds__(256)
   for (int j = 0; j < PER; ++j) {
     const int e = lane + 64 * j;
     valid[j] = e < c;
-    bits[j] = valid[j] ? __float_as_uint(cand[(size_t)row * kCandPerRow + e]) : 0u;
+    bits[j] = valid[j] ? __float_as_uint(cand[(size_t)row * kCap + e]) : 0u;
   }
   unsigned int prefix = 0, mask = 0;
   int remaining = k;
@@ -495,10 +507,11 @@ int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, 
   s(mlpForward(params, feats, n, pred));
   hipLaunchKernelGGL(radiusKernel, dim3((n + 255) / 256), dim3(256), 0, stream(), pred.get(), n,
                      (float)tsd, (float)(tm + margin), radius.get());
-  s(cand.alloc((size_t)n * kCandPerRow));
+  const int cap = candSlotsFor(k);
+  s(cand.alloc((size_t)n * cap));
   if (keep)
-    s(lists.pos.alloc((size_t)n * kCandPerRow));
-  lists.cap = kCandPerRow;
+    s(lists.pos.alloc((size_t)n * cap));
+  lists.cap = cap;
   s(fill(numFallback, 0, sizeof(int)));
   s(fill(counters, 0, counters.bytes()));
   if (!s.ok())
@@ -509,15 +522,23 @@ int coreLearned(const float *X, int n, int d, MetricSpec m, int k, float *core, 
       using T = Tile<8, decltype(fill)::value>;
       hipLaunchKernelGGL((radiusGatherKernel<T, Metric>), dim3(tiles), dim3(256),
                          2 * d * sizeof(float), stream(), X, n, d, metric, radius.get(),
-                         low.get(), high.get(), cand.get(), lists.pos.get(), count.get(),
+                         low.get(), high.get(), cand.get(), lists.pos.get(), count.get(), cap,
                          counters.get());
       return (int)hipGetLastError();
     };
     return d <= 8 ? run(std::integral_constant<int, 8>{}) : run(std::integral_constant<int, 32>{});
   }));
-  hipLaunchKernelGGL(candidateSelectKernel, dim3((n + 3) / 4), dim3(256), 0, stream(), cand.get(),
-                     count.get(), n, k, core, fallback.get(), numFallback.get(),
-                     counters.get() + 1);
+  auto select = [&](auto per) {
+    hipLaunchKernelGGL((candidateSelectKernel<decltype(per)::value>), dim3((n + 3) / 4),
+                       dim3(256), 0, stream(), cand.get(), count.get(), n, k, core,
+                       fallback.get(), numFallback.get(), counters.get() + 1);
+  };
+  switch (cap) {
+  case 512: select(std::integral_constant<int, 8>{}); break;
+  case 1024: select(std::integral_constant<int, 16>{}); break;
+  case 2048: select(std::integral_constant<int, 32>{}); break;
+  default: select(std::integral_constant<int, 64>{}); break;
+  }
   s((int)hipGetLastError());
   int nf = 0;
   s(download(&nf, numFallback, sizeof(int)));

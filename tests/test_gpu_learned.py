@@ -219,3 +219,24 @@ def test_learned_lists_off_for_scaled_reachability(alpha):
     np.testing.assert_array_equal(core, co)
     np.testing.assert_array_equal(Z, Zo)
     assert st["settled_rows"] == 0 and st["offered_rows"] == 0
+
+
+def test_auto_uses_the_learned_radius_above_k64():
+    # For k > 64 on large inputs auto hands the core distances to the learned
+    # radius (the alternative is the full-row radix select); the result is the
+    # radix select's bit for bit, and it gathers far fewer candidates than the
+    # n^2 distances the radix select writes and reads.
+    rng = np.random.default_rng(21)
+    parts = [rng.normal(scale=s, size=(30_000, 5)) + rng.uniform(-40, 40, 5)
+             for s in [0.2, 0.5, 1.0, 2.0]]
+    parts.append(rng.uniform(-45, 45, (20_000, 5)))
+    X = np.concatenate(parts).astype(np.float32)
+    n, k = len(X), 100
+    _lib.learned_stats()
+    _, core_auto = hdbscan_tree(X, k)
+    st = _lib.learned_stats()
+    _, core_radix = hdbscan_tree(X, k, core_method="radix")
+    np.testing.assert_array_equal(core_auto, core_radix)
+    print(st)
+    assert st["proposals"] == n
+    assert st["candidates"] < 0.05 * n * n
