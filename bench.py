@@ -81,6 +81,29 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_kib_per_launch": nbytes >> 10, "launches": 2}), flush=True)
 
 
+# MFMA probe shapes (cuhc_mfma_probe): what the candidate pass's instruction
+# mix can issue without memory traffic, beside the data-sheet peak.
+MFMA_PROBE_SHAPES = {0: "32x32x16 f16, accumulate in place",
+                     1: "16x16x32 f16, accumulate in place",
+                     2: "candidate loop: seeded 32x32x16, LDS operands, min epilogue",
+                     3: "candidate loop on 16x16x32: seeded, LDS operands, min epilogue"}
+
+
+def mfma_ceiling(workgroups=2048, iters=4096):
+    """TFLOP/s of each probe shape (three timed launches each, HIP events)."""
+    import ctypes
+    from cuhc_amd import _lib
+    out = {}
+    for shape, name in MFMA_PROBE_SHAPES.items():
+        ms, flops = ctypes.c_float(), ctypes.c_double()
+        _lib.check(_lib.lib.cuhc_mfma_probe(shape, workgroups, iters, ctypes.byref(ms),
+                                            ctypes.byref(flops)))
+        tf = flops.value / (ms.value * 1e-3) / 1e12
+        out[name] = {"ms": round(ms.value, 3), "tflops": round(tf, 1),
+                     "frac_of_peak": round(tf / FP16_MFMA_PEAK_TFLOPS, 3)}
+    return out
+
+
 def timed_steps(fn, steps):
     # Each call is blocking: it returns after the tree is in host memory and
     # the library's stream is idle, so wall time brackets exactly the steps.
@@ -316,6 +339,8 @@ def main():
     parser.add_argument("--print-maps", action="store_true",
                         help="print the memory-map lines of libcuhc.so and the HIP runtime "
                              "after loading (to attribute a crash address)")
+    parser.add_argument("--mfma-ceiling", action="store_true",
+                        help="only run the MFMA issue-rate probe (attainable FP16 ceiling)")
     parser.add_argument("--calibrate-copy-mib", type=int, default=0,
                         help="only run a device-to-device copy of this size (counter scaling)")
     args = parser.parse_args()
@@ -335,6 +360,9 @@ def main():
 
     if args.calibrate_copy_mib > 0:
         calibrate_copy(args.calibrate_copy_mib)
+        return
+    if args.mfma_ceiling:
+        print(json.dumps({"mfma_ceiling": mfma_ceiling()}), flush=True)
         return
     if args.config != "c4":
         other_config(args.config, args)
@@ -386,6 +414,16 @@ def main():
                               if v[1]},
         "mixed_stats_per_tree": {k: v / max(args.steps, 1) for k, v in stats.items()},
     }
+
+    # The same instruction mix without memory traffic or records (the probe's
+    # candidate-loop shape) on this GPU: an attainable ceiling beside the
+    # data-sheet peak.
+    ceiling = mfma_ceiling()
+    attainable = ceiling[MFMA_PROBE_SHAPES[2]]["tflops"]
+    result["roofline"]["attainable"] = attainable
+    result["roofline"]["frac_of_attainable"] = round(achieved / attainable, 4)
+    result["mfma_probe"] = ceiling
+    log(f"mfma probe: {ceiling}")
 
     if args.plain_steps > 0:
         _lib.set_precision("fp32")

@@ -70,6 +70,8 @@ _declare("cuhc_device_free", _int, ctypes.c_void_p)
 _declare("cuhc_copy_to_device", _int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong)
 _declare("cuhc_copy_device", _int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong)
 _declare("cuhc_copy_to_host", _int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong)
+_declare("cuhc_mfma_probe", _int, _int, _int, _int, ctypes.POINTER(ctypes.c_float),
+         ctypes.POINTER(ctypes.c_double))
 _declare("cuhc_set_timing", _int, _int)
 _declare("cuhc_read_timings", None, _f64p, _i64p, _i64p, _int)
 _declare("cuhc_reset_timings", None)

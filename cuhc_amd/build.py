@@ -38,12 +38,12 @@ DEVICE = [
 
 SOURCES = [
     "runtime.hip", "primitives.hip", "distance.hip", "boruvka.hip", "core.hip", "dense.hip", "ward.hip",
-    "mfma.hip", "mlp.hip", "learned.hip", "edge_sort.hip", "contract.hip", "capi.hip", "noalloc.hip", "host_tree.cpp", "extract.cpp",
+    "mfma.hip", "mfma_probe.hip", "mlp.hip", "learned.hip", "edge_sort.hip", "contract.hip", "capi.hip", "noalloc.hip", "host_tree.cpp", "extract.cpp",
 ]
 
 # Per-file extra flags. mfma.hip: inputs are validated finite, so min/max need
 # no NaN canonicalisation (the epilogue is one v_min per value).
-EXTRA = {"mfma.hip": ["-fno-honor-nans"]}
+EXTRA = {"mfma.hip": ["-fno-honor-nans"], "mfma_probe.hip": ["-fno-honor-nans"]}
 
 NOISE = "is not a recognized feature for this target"
 

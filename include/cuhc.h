@@ -161,6 +161,17 @@ int cuhc_copy_to_host(void *dst, const void *src, unsigned long long bytes);
  * uses it to scale the HBM byte counters). No counterpart in the reference. */
 int cuhc_copy_device(void *dst, const void *src, unsigned long long bytes);
 
+/* Matrix-core issue-rate probe: times `workgroups` workgroups of 256 threads
+ * running `iters` steps of one instruction mix (shape 0: 32x32x16 FP16 MFMAs
+ * accumulating in place; 1: 16x16x32 FP16 MFMAs accumulating in place; 2: the
+ * candidate pass's loop on synthetic LDS operands, seeded accumulators and the
+ * per-lane minimum epilogue; 3: that loop on 16x16x32 MFMAs) and returns ms
+ * and flops per launch. No memory
+ * traffic; the benchmark uses it to state an attainable MFMA ceiling. No
+ * counterpart in the reference. */
+int cuhc_mfma_probe(int shape, int workgroups, int iters, float *msPerLaunch,
+                    double *flopsPerLaunch);
+
 /* Frees what the library keeps between calls: its cache of freed device
  * blocks, the pinned staging buffer and its HIP stream (recreated on the next
  * call). Also runs from an exit handler, before the HIP runtime's teardown. */
