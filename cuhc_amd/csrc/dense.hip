@@ -14,18 +14,25 @@
 // the row's minimum; row a's new fn is the minimum of its new distances.
 //
 // The merge loop is one cooperative persistent kernel (one workgroup per CU)
-// with two grid barriers per merge. Workgroup g owns a contiguous range of
-// slots (rows) and is the only writer of their fn, size, id and active flag,
-// and of D(k, a) / D(a, k) for its rows k. Merge s:
-//   update:  every workgroup reduces the per-workgroup selection records to
-//            the pair (a, b); applies the Lance-Williams update to D(a, k) for
-//            its own k, maintains its rows' fn, lists its rows that lost their
-//            nearest, and publishes its part of row a's new fn;
-//   barrier;
-//   rescan:  the owner of a reduces row a's parts; all workgroups share the
-//            listed rows' rescans in column chunks (atomicMin into fn), and
-//            each publishes the minimum (selection key, fn) over its own rows
-//            and the chunks it scanned;
+// with one grid barrier per merge. Workgroup g owns a contiguous range of
+// slots (rows): their fn and size live in its LDS, it alone writes D(k, a) /
+// D(a, k) for its rows k, and every workgroup keeps the active flags of all
+// slots as a bitmask in LDS. Phase s (between two barriers):
+//   select:  every workgroup reads all records of phase s - 1 and reduces them
+//            the same way: the previous merged row's new fn (the minimum of
+//            the workgroups' parts) and the pair (a, b) = the lowest
+//            (selection key, fn) over every row, that one included; b's bit
+//            is cleared; the owner of the previous merged slot installs its
+//            fn, size and id (sizes and ids change one phase late, so no read
+//            in the merge's own phase races the write);
+//   update:  Lance-Williams update of D(a, k) for its own k, its rows' fn,
+//            and its part of row a's new fn (minimum over its k > a);
+//   rescan:  its rows whose nearest was a or b and is not the merged distance
+//            are scanned again by the whole workgroup (no other workgroup
+//            changes an entry of such a row in this phase, and D(k, a) is the
+//            value it just stored);
+//   record:  the minimum (selection key, fn) over its rows but a, and its part
+//            of row a's fn;
 //   barrier.
 // Per-workgroup records replace same-address atomics, which serialize across
 // workgroups. The tree row is written on the device; there is no host round
@@ -43,9 +50,9 @@ constexpr int kThreads = 512;
 constexpr int kChunk = 8192;                     // columns per initial-scan work item
 constexpr int kRescanChunk = 4096;               // columns per rescan work item
 constexpr int kRescanPer = kRescanChunk / kThreads;
-constexpr int kMaxOwned = 2048;                  // rows per workgroup (LDS rescan list)
-constexpr int kInlineRescans = 5;                // rescan rows carried in an update record
-constexpr int kRecordInts = 8;                   // update record: count, rows, padding
+constexpr int kMaxOwned = 2048;                  // rows per workgroup (LDS row state)
+constexpr int kRecordWords = 8;                  // u64 words per workgroup record
+constexpr int kInlineRows = 6;                   // pending rows carried in a record
 constexpr int kGroup = 16;                       // workgroups per first-level barrier counter
 constexpr int kLineWords = 32;                   // one 128-byte line in 32-bit words
 constexpr unsigned int kSpinLimit = 1u << 22;    // barrier polls (seconds) before aborting
@@ -147,20 +154,27 @@ struct DenseState {
   int n;
   int linkage;
   int firstNewId;
+  // Cluster size and scipy id by slot. The owner of a merged slot writes its
+  // new size and id one phase after the merge (every workgroup still reads
+  // the old size in the merge's own phase, and knows the new one itself).
   int *size;
   int *id;
-  unsigned char *active;
-  unsigned long long *fnKey;       // per slot: denseKey(D(i, fn(i)), fn(i)) or empty
-  unsigned long long *selectSlots; // [2 G]: per workgroup (selection key, fn) of its best row
-  // Update records, one per workgroup: the minimum over its k > a of
-  // (D(a, k), k), and its rows to rescan (count, then up to kInlineRescans
-  // rows; all of them in rescanRows[g kMaxOwned ...] when there are more).
-  unsigned long long *rowASlots;   // [G]
-  int *records;                    // [G kRecordInts]
-  int *rescanRows;                 // [G kMaxOwned]
-  int2 *items;                     // initial fn: (row, first column) work items
-  double *tree;                    // (n - 1) x 4, device
-  unsigned long long *clocks;      // null, or [4]: device-clock ticks per part of a merge
+  unsigned long long *fnKey;    // initial forward nearest per slot (denseInitKernel)
+  unsigned long long *fnRescan; // per slot: the rescan's minimum (atomicMin) while pending
+  // One record per workgroup and publish (kRecordWords words): [0] and [1]
+  // the minimum (selection key, fn) over its settled rows, [2] its part of
+  // the merged row's new fn, [3] the lowest lower-bound key of its pending
+  // rows, [4] the number of rows it made pending in the phase, [5..7] the
+  // first kInlineRows of them (two per word); all of them in pendingRows[g
+  // kMaxOwned ...] when there are more. Two buffers, alternating by publish:
+  // consecutive publishes are separated by a grid barrier, so a buffer is
+  // rewritten only after every workgroup has passed the barrier that follows
+  // its last reader's next publish.
+  unsigned long long *records; // [2][G kRecordWords]
+  int *pendingRows;            // [2][G kMaxOwned]
+  int2 *items;                 // initial fn: (row, first column) work items
+  double *tree;                // (n - 1) x 4, device
+  unsigned long long *clocks;  // null, or [4]: device-clock ticks per part of a merge
   Barrier bar;
 };
 
@@ -230,36 +244,14 @@ DEV void blockMinPair(unsigned long long &k, unsigned long long &p, unsigned lon
     pairMin(k, p, redK[w], redP[w]);
 }
 
-// Minimum over active j in [j0, j1) of denseKey(D(row, j), j), by the whole
-// workgroup.
+// Minimum over j in [j0, j1) of denseKey(D(row, j), j), by the whole
+// workgroup (every slot is active before the first merge).
 DEV unsigned long long scanRowSegment(const DenseState &s, int row, int j0, int j1,
                                       unsigned long long *red) {
   unsigned long long best = kEmptyKey;
   const size_t base = condensedIndex(row, row + 1, s.n) - (size_t)(row + 1);
   for (int j = j0 + (int)threadIdx.x; j < j1; j += kThreads)
-    if (s.active[j])
-      best = minKey(best, denseKey(s.D[base + j], (unsigned)j));
-  return blockMinKey(best, red);
-}
-
-// The same over one rescan chunk (at most kRescanChunk columns), with every
-// load issued before the first compare.
-DEV unsigned long long scanRescanChunk(const DenseState &s, int row, int j0, int j1,
-                                       unsigned long long *red) {
-  const size_t base = condensedIndex(row, row + 1, s.n) - (size_t)(row + 1);
-  float v[kRescanPer];
-  unsigned char act[kRescanPer];
-#pragma unroll
-  for (int u = 0; u < kRescanPer; ++u) {
-    const int j = j0 + (int)threadIdx.x + u * kThreads;
-    act[u] = j < j1 ? s.active[j] : 0;
-    v[u] = j < j1 ? s.D[base + j] : 0.0f;
-  }
-  unsigned long long best = kEmptyKey;
-#pragma unroll
-  for (int u = 0; u < kRescanPer; ++u)
-    if (act[u])
-      best = minKey(best, denseKey(v[u], (unsigned)(j0 + (int)threadIdx.x + u * kThreads)));
+    best = minKey(best, denseKey(s.D[base + j], (unsigned)j));
   return blockMinKey(best, red);
 }
 
@@ -280,18 +272,57 @@ DEV unsigned long long selectionKey(unsigned long long fn, int row) {
   return fn == kEmptyKey ? kEmptyKey : (fn & 0xffffffff00000000ull) | (unsigned int)row;
 }
 
+DEV bool isActive(const unsigned int *activeBits, int j) {
+  return (activeBits[j >> 5] >> (j & 31)) & 1u;
+}
+
+// Minimum over active j in [j0, j1) (at most kRescanChunk columns), j != skip,
+// of denseKey(D(row, j), j), by the whole workgroup, every load issued
+// before the first compare.
+DEV unsigned long long scanRescanChunk(const DenseState &s, const unsigned int *activeBits,
+                                       int row, int j0, int j1, int skip,
+                                       unsigned long long *red) {
+  const size_t base = condensedIndex(row, row + 1, s.n) - (size_t)(row + 1);
+  float v[kRescanPer];
+#pragma unroll
+  for (int u = 0; u < kRescanPer; ++u) {
+    const int j = j0 + (int)threadIdx.x + u * kThreads;
+    v[u] = j < j1 ? s.D[base + j] : 0.0f;
+  }
+  unsigned long long best = kEmptyKey;
+#pragma unroll
+  for (int u = 0; u < kRescanPer; ++u) {
+    const int j = j0 + (int)threadIdx.x + u * kThreads;
+    if (j < j1 && j != skip && isActive(activeBits, j))
+      best = minKey(best, denseKey(v[u], (unsigned)j));
+  }
+  return blockMinKey(best, red);
+}
+
 __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
-  __shared__ unsigned long long redK[kThreads / 64], redP[kThreads / 64];
-  __shared__ int ownRescan[kMaxOwned];
+  extern __shared__ unsigned int activeBits[]; // (n + 31) / 32 words: bit j = slot j active
+  __shared__ unsigned long long ownFn[kMaxOwned];
+  __shared__ int ownSize[kMaxOwned];
+  __shared__ unsigned char ownPending[kMaxOwned];
+  // This workgroup's pending rows of the two latest phases and their
+  // lower-bound keys: list[older] is being (or was) rescanned, list[older ^ 1]
+  // is filled by the current update.
+  __shared__ int pendRow[2][kMaxOwned];
+  __shared__ unsigned long long pendKey[2][kMaxOwned];
+  __shared__ int pendCount[2];
+  // The rows every workgroup made pending in the previous phase:
+  // workgroup w's at [listStart[w], listStart[w + 1]).
   __shared__ int listStart[kThreads + 1];
-  __shared__ int listRows[kInlineRescans * kThreads];
+  __shared__ int listRows[kInlineRows * kThreads];
   __shared__ int waveSums[kThreads / 64];
-  __shared__ int ownRescanCount, barrierOk;
+  __shared__ unsigned long long redK[kThreads / 64], redP[kThreads / 64], redA[kThreads / 64];
+  __shared__ int barrierOk;
   const int n = s.n;
   const int G = gridDim.x, g = blockIdx.x;
   const int tid = threadIdx.x;
   const int per = (n + G - 1) / G;
   const int mine0 = min(n, g * per), mine1 = min(n, mine0 + per);
+  const int words = (n + 31) >> 5;
   const int maxChunks = (n - 1 + kRescanChunk - 1) / kRescanChunk;
   unsigned int generation = 0;
   const bool clocked = s.clocks && g == 0 && tid == 0;
@@ -303,77 +334,245 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
       t0 = t;
     }
   };
+  int published = 0; // publishes so far: the latest is in buffer (published - 1) & 1
+  auto recordsOf = [&](int buffer) { return s.records + (size_t)buffer * G * kRecordWords; };
+  auto pendingOf = [&](int buffer) { return s.pendingRows + (size_t)buffer * G * kMaxOwned; };
+  // Publishes this workgroup's record: settled rows, row-a part, pending
+  // lower bound, and the rows made pending in this phase (list[older ^ 1]).
+  auto publish = [&](unsigned long long selK, unsigned long long selP,
+                     unsigned long long rowA, int older) {
+    const int buffer = published & 1;
+    unsigned long long *record = recordsOf(buffer) + (size_t)g * kRecordWords;
+    unsigned long long pmin = kEmptyKey;
+    for (int l = 0; l < 2; ++l)
+      for (int i = tid; i < pendCount[l]; i += kThreads)
+        if (isActive(activeBits, pendRow[l][i]))
+          pmin = minKey(pmin, pendKey[l][i]);
+    pmin = blockMinKey(pmin, redA);
+    rowA = blockMinKey(rowA, redK);
+    blockMinPair(selK, selP, redK, redP);
+    const int fresh = older ^ 1, count = pendCount[fresh];
+    if (count > kInlineRows)
+      for (int i = tid; i < count; i += kThreads)
+        pendingOf(buffer)[(size_t)g * kMaxOwned + i] = pendRow[fresh][i];
+    if (tid == 0) {
+      record[0] = selK;
+      record[1] = selP;
+      record[2] = rowA;
+      record[3] = pmin;
+      record[4] = (unsigned long long)count;
+      for (int w = 0; w < kInlineRows / 2; ++w) {
+        const int r0 = 2 * w < count ? pendRow[fresh][2 * w] : 0;
+        const int r1 = 2 * w + 1 < count ? pendRow[fresh][2 * w + 1] : 0;
+        record[5 + w] = (unsigned long long)(unsigned)r0 | ((unsigned long long)(unsigned)r1 << 32);
+      }
+    }
+    ++published;
+  };
+  // The q-th row of the previous phase's pending list (listStart ready).
+  auto listRow = [&](int q, bool spilled) {
+    if (!spilled)
+      return listRows[q];
+    int lo = 0, hi = G; // the w with listStart[w] <= q < listStart[w + 1]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (listStart[mid] <= q)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const int i = q - listStart[lo];
+    const int buffer = (published - 1) & 1;
+    if (listStart[lo + 1] - listStart[lo] > kInlineRows)
+      return pendingOf(buffer)[(size_t)lo * kMaxOwned + i];
+    const unsigned long long w = recordsOf(buffer)[(size_t)lo * kRecordWords + 5 + i / 2];
+    return (int)(unsigned)(i & 1 ? w >> 32 : w);
+  };
+  // Rescans of the listed rows, shared by all workgroups in column chunks:
+  // item t is chunk t % maxChunks of list row t / maxChunks. Column skip
+  // (the merged slot, whose entries their owners are rewriting) is left out.
+  auto rescans = [&](int count, bool spilled, int skip) {
+    const long long numItems = (long long)count * maxChunks;
+    for (long long t = g; t < numItems; t += G) {
+      const int r = listRow((int)(t / maxChunks), spilled);
+      const int j0 = r + 1 + (int)(t % maxChunks) * kRescanChunk;
+      if (j0 >= n || !isActive(activeBits, r))
+        continue;
+      const unsigned long long m =
+          scanRescanChunk(s, activeBits, r, j0, min(n, j0 + kRescanChunk), skip, redK);
+      if (tid == 0 && m != kEmptyKey)
+        atomicMin(&s.fnRescan[r], m);
+    }
+  };
+  // Settles this workgroup's rows of list[l]: their fn is the rescan's result.
+  auto install = [&](int l) {
+    for (int i = tid; i < pendCount[l]; i += kThreads) {
+      const int r = pendRow[l][i];
+      ownPending[r - mine0] = 0;
+      ownFn[r - mine0] = isActive(activeBits, r) ? s.fnRescan[r] : kEmptyKey;
+    }
+    __syncthreads();
+    if (tid == 0)
+      pendCount[l] = 0;
+  };
+  // The settled rows' minimum (selection key, fn), all rows but skipRow.
+  auto settledMin = [&](unsigned long long &k, unsigned long long &p, int skipRow) {
+    for (int i = mine0 + tid; i < mine1; i += kThreads)
+      if (i != skipRow && !ownPending[i - mine0] && isActive(activeBits, i))
+        pairMin(k, p, selectionKey(ownFn[i - mine0], i), ownFn[i - mine0]);
+  };
 
-  // Selection records of merge 0 from the initial fn.
+  // Every slot starts active and settled; this workgroup's rows' state moves
+  // to LDS, and the records of merge 0 come from the initial fn.
+  for (int w = tid; w < words; w += kThreads) {
+    const int rest = n - 32 * w;
+    activeBits[w] = rest >= 32 ? 0xffffffffu : (1u << rest) - 1u;
+  }
+  for (int i = mine0 + tid; i < mine1; i += kThreads) {
+    ownFn[i - mine0] = s.fnKey[i];
+    ownSize[i - mine0] = s.size[i];
+    ownPending[i - mine0] = 0;
+  }
+  if (tid < 2)
+    pendCount[tid] = 0;
+  __syncthreads();
+  int older = 0;
   {
     unsigned long long k = kEmptyKey, p = kEmptyKey;
-    for (int i = mine0 + tid; i < mine1; i += kThreads) {
-      const unsigned long long f = s.fnKey[i];
-      pairMin(k, p, selectionKey(f, i), f);
-    }
-    blockMinPair(k, p, redK, redP);
-    if (tid == 0) {
-      s.selectSlots[2 * g] = k;
-      s.selectSlots[2 * g + 1] = p;
-    }
+    settledMin(k, p, -1);
+    publish(k, p, kEmptyKey, older);
   }
   if (!gridBarrier(s.bar, generation, &barrierOk))
     return;
   if (clocked)
     t0 = wall_clock64();
 
+  int prevA = -1, prevSize = 0; // the previous merge's slot a and merged size
   for (int step = 0; step < n - 1; ++step) {
-    // The pair: the lowest (selection key, fn) record.
-    if (tid == 0)
-      ownRescanCount = 0;
-    unsigned long long best = kEmptyKey, fa = kEmptyKey;
-    for (int t = tid; t < G; t += kThreads)
-      pairMin(best, fa, s.selectSlots[2 * t], s.selectSlots[2 * t + 1]);
-    blockMinPair(best, fa, redK, redP);
-    const int a = best == kEmptyKey ? -1 : keyIndex(best);
-    const int b = fa == kEmptyKey ? -1 : keyIndex(fa);
+    // Records of the previous phase: every workgroup reduces them the same
+    // way. The previous merged row's new fn is the minimum of the parts.
+    unsigned long long k = kEmptyKey, p = kEmptyKey, part = kEmptyKey, pmin = kEmptyKey;
+    int cnt = 0;
+    int inl[kInlineRows];
+    if (tid < G) {
+      const unsigned long long *r = recordsOf((published - 1) & 1) + (size_t)tid * kRecordWords;
+      k = r[0];
+      p = r[1];
+      part = r[2];
+      pmin = r[3];
+      cnt = (int)r[4];
+      for (int w = 0; w < kInlineRows / 2; ++w) {
+        const unsigned long long x = r[5 + w];
+        inl[2 * w] = (int)(unsigned)x;
+        inl[2 * w + 1] = (int)(unsigned)(x >> 32);
+      }
+    }
+    part = blockMinKey(part, redA);
+    pmin = blockMinKey(pmin, redK);
+    blockMinPair(k, p, redK, redP);
+    if (prevA >= 0)
+      pairMin(k, p, selectionKey(part, prevA), part);
+    int listCount = 0;
+    const int start = blockExclusiveScan(cnt, waveSums, listCount);
+    const bool spilled = __syncthreads_or(cnt > kInlineRows);
+    if (tid < G) {
+      listStart[tid] = start;
+      if (!spilled)
+        for (int i = 0; i < cnt; ++i)
+          listRows[start + i] = inl[i];
+    }
+    if (tid == 0) {
+      listStart[G] = listCount;
+      if (prevA >= mine0 && prevA < mine1) {
+        ownFn[prevA - mine0] = part;
+        ownSize[prevA - mine0] = prevSize;
+        s.size[prevA] = prevSize;
+        s.id[prevA] = s.firstNewId + step - 1;
+      }
+    }
+    // Rows rescanned in the previous phase settle; this phase's list is new.
+    install(older);
+    older ^= 1;
+    bool rescanned = false;
+    if (!(k < pmin)) {
+      // A pending row could hold the lowest key: rescan the previous phase's
+      // rows now, settle every pending row, and select again from records
+      // that hold no pending row (two extra barriers; rare).
+      rescans(listCount, spilled, -1);
+      if (!gridBarrier(s.bar, generation, &barrierOk))
+        return;
+      install(older);
+      unsigned long long k2 = kEmptyKey, p2 = kEmptyKey;
+      settledMin(k2, p2, -1);
+      publish(k2, p2, kEmptyKey, older);
+      if (!gridBarrier(s.bar, generation, &barrierOk))
+        return;
+      k = kEmptyKey;
+      p = kEmptyKey;
+      if (tid < G) {
+        const unsigned long long *r = recordsOf((published - 1) & 1) + (size_t)tid * kRecordWords;
+        k = r[0];
+        p = r[1];
+      }
+      blockMinPair(k, p, redK, redP);
+      rescanned = true;
+    }
+    const int a = k == kEmptyKey ? -1 : keyIndex(k);
+    const int b = p == kEmptyKey ? -1 : keyIndex(p);
     if (a < 0 || b <= a || b >= n) {
       // No pair (non-finite distances): the remaining rows report -1. Every
-      // workgroup read the same records, so all of them stop here.
+      // workgroup reduced the same records, so all of them stop here.
       if (g == 0)
         for (int r = step + tid; r < n - 1; r += kThreads)
           for (int c = 0; c < 4; ++c)
             s.tree[(size_t)r * 4 + c] = -1.0;
       break;
     }
-    const float dab = fromOrderedBits((unsigned)(fa >> 32));
-    const int na = s.size[a], nb = s.size[b];
+    const float dab = fromOrderedBits((unsigned)(p >> 32));
+    // Sizes and ids in memory are at least one phase old; the previous
+    // merge's cluster is known here.
+    const int na = a == prevA ? prevSize : s.size[a];
+    const int nb = b == prevA ? prevSize : s.size[b];
     if (g == 0 && tid == 0) {
-      const int ia = s.id[a], ib = s.id[b];
+      const int prevId = s.firstNewId + step - 1;
+      const int ia = a == prevA ? prevId : s.id[a], ib = b == prevA ? prevId : s.id[b];
       double *row = s.tree + (size_t)step * 4;
       row[0] = (double)min(ia, ib);
       row[1] = (double)max(ia, ib);
       row[2] = (double)dab;
       row[3] = (double)(na + nb);
     }
+    if (tid == 0)
+      activeBits[b >> 5] &= ~(1u << (b & 31));
+    __syncthreads();
     lap(0);
 
-    // Lance-Williams update of D(a, k) for this workgroup's k, its rows' fn,
-    // and its part of row a's new fn.
+    // Lance-Williams update of D(a, k) for this workgroup's k, its settled
+    // rows' fn, and its part of row a's new fn.
     unsigned long long rowAMin = kEmptyKey, selK = kEmptyKey, selP = kEmptyKey;
-    for (int k = mine0 + tid; k < mine1; k += kThreads) {
-      if (k == a || k == b)
+    const int fresh = older ^ 1;
+    for (int kk = mine0 + tid; kk < mine1; kk += kThreads) {
+      if (kk == a || kk == b || !isActive(activeBits, kk))
         continue;
-      // Every load is issued before the activity test: one round trip.
-      const size_t iak = k < a ? condensedIndex(k, a, n) : condensedIndex(a, k, n);
-      const size_t ibk = k < b ? condensedIndex(k, b, n) : condensedIndex(b, k, n);
-      const unsigned char act = s.active[k];
+      const size_t iak = kk < a ? condensedIndex(kk, a, n) : condensedIndex(a, kk, n);
+      const size_t ibk = kk < b ? condensedIndex(kk, b, n) : condensedIndex(b, kk, n);
       const float dak = s.D[iak], dbk = s.D[ibk];
-      const unsigned long long f = s.fnKey[k];
-      const int nk = s.size[k];
-      if (!act)
-        continue;
-      const float nd = (float)lanceWilliams(s.linkage, dak, dbk, dab, na, nb, nk);
+      const unsigned long long f = ownFn[kk - mine0];
+      const float nd =
+          (float)lanceWilliams(s.linkage, dak, dbk, dab, na, nb, ownSize[kk - mine0]);
       s.D[iak] = nd;
+      if (kk > a)
+        rowAMin = minKey(rowAMin, denseKey(nd, (unsigned)kk));
+      if (ownPending[kk - mine0]) {
+        // Being rescanned without column a: its new D(kk, a) joins here.
+        if (kk < a)
+          atomicMin(&s.fnRescan[kk], denseKey(nd, (unsigned)a));
+        continue;
+      }
       unsigned long long nf = f;
       bool rescan = false;
-      if (k < a) {
-        // Every entry of row k but D(k, a) and D(k, b) is unchanged.
+      if (kk < a) {
+        // Every entry of row kk but D(kk, a) and D(kk, b) is unchanged.
         const int fk = f == kEmptyKey ? -1 : keyIndex(f);
         const unsigned long long cand = denseKey(nd, (unsigned)a);
         if (fk == a || fk == b) {
@@ -386,114 +585,34 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
         } else if (cand < f) {
           nf = cand;
         }
-      } else {
-        rowAMin = minKey(rowAMin, denseKey(nd, (unsigned)k));
-        if (k < b && f != kEmptyKey && keyIndex(f) == b)
-          rescan = true; // its nearest retired
+      } else if (kk < b && f != kEmptyKey && keyIndex(f) == b) {
+        rescan = true; // its nearest retired
       }
       if (rescan) {
-        nf = kEmptyKey; // rebuilt by atomicMin over the chunks
-        ownRescan[atomicAdd(&ownRescanCount, 1)] = k;
+        // Pending until the next phase's rescan; its new key is at least
+        // the old one (every linkage here is reducible).
+        const int r = atomicAdd(&pendCount[fresh], 1);
+        pendRow[fresh][r] = kk;
+        pendKey[fresh][r] = selectionKey(f, kk);
+        ownPending[kk - mine0] = 1;
+        s.fnRescan[kk] = kEmptyKey;
+        nf = kEmptyKey;
       } else {
-        pairMin(selK, selP, selectionKey(nf, k), nf);
+        pairMin(selK, selP, selectionKey(nf, kk), nf);
       }
-      if (nf != f)
-        s.fnKey[k] = nf;
+      ownFn[kk - mine0] = nf;
     }
-    rowAMin = blockMinKey(rowAMin, redK); // also orders the ownRescan appends
-    const int own = ownRescanCount;
-    int *record = s.records + (size_t)g * kRecordInts;
-    if (tid == 0) {
-      s.rowASlots[g] = rowAMin;
-      record[0] = own;
-    }
-    if (tid < min(own, kInlineRescans))
-      record[1 + tid] = ownRescan[tid];
-    if (own > kInlineRescans)
-      for (int i = tid; i < own; i += kThreads)
-        s.rescanRows[(size_t)g * kMaxOwned + i] = ownRescan[i];
-    if (tid == 0 && b >= mine0 && b < mine1) {
-      s.active[b] = 0;
-      s.fnKey[b] = kEmptyKey;
-    }
-    lap(1);
-    if (!gridBarrier(s.bar, generation, &barrierOk))
-      return;
-    lap(2);
-
-    // Every workgroup's update record (G <= kThreads: one per thread).
-    unsigned long long rowAPart = kEmptyKey;
-    int cnt = 0;
-    int inlineRows[kInlineRescans];
-    if (tid < G) {
-      rowAPart = s.rowASlots[tid];
-      const int4 r0 = *reinterpret_cast<const int4 *>(s.records + (size_t)tid * kRecordInts);
-      const int4 r1 = *reinterpret_cast<const int4 *>(s.records + (size_t)tid * kRecordInts + 4);
-      cnt = r0.x;
-      inlineRows[0] = r0.y;
-      inlineRows[1] = r0.z;
-      inlineRows[2] = r0.w;
-      inlineRows[3] = r1.x;
-      inlineRows[4] = r1.y;
-    }
-    // Row a's new fn, from every workgroup's part.
-    if (a >= mine0 && a < mine1) {
-      const unsigned long long m = blockMinKey(rowAPart, redK);
-      if (tid == 0) {
-        s.fnKey[a] = m;
-        s.size[a] = na + nb;
-        s.id[a] = s.firstNewId + step;
-        pairMin(selK, selP, selectionKey(m, a), m);
-      }
-    }
-    // The rescan list: workgroup w's rows at [listStart[w], listStart[w + 1]).
-    int count = 0;
-    const int start = blockExclusiveScan(cnt, waveSums, count);
-    const bool spilled = __syncthreads_or(cnt > kInlineRescans);
-    if (tid < G) {
-      listStart[tid] = start;
-      if (!spilled)
-        for (int i = 0; i < cnt; ++i)
-          listRows[start + i] = inlineRows[i];
-    }
-    if (tid == 0)
-      listStart[G] = count;
+    if (tid == 0 && b >= mine0 && b < mine1)
+      ownFn[b - mine0] = kEmptyKey;
     __syncthreads();
-    // Rescans: item t is chunk t % maxChunks of list row t / maxChunks.
-    const long long numItems = (long long)count * maxChunks;
-    for (long long t = g; t < numItems; t += G) {
-      const int q = (int)(t / maxChunks);
-      int r;
-      if (!spilled) {
-        r = listRows[q];
-      } else {
-        int lo = 0, hi = G; // the w with listStart[w] <= q < listStart[w + 1]
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (listStart[mid] <= q)
-            lo = mid;
-          else
-            hi = mid;
-        }
-        const int i = q - listStart[lo];
-        r = listStart[lo + 1] - listStart[lo] <= kInlineRescans
-                ? s.records[(size_t)lo * kRecordInts + 1 + i]
-                : s.rescanRows[(size_t)lo * kMaxOwned + i];
-      }
-      const int j0 = r + 1 + (int)(t % maxChunks) * kRescanChunk;
-      if (j0 >= n)
-        continue;
-      const unsigned long long m = scanRescanChunk(s, r, j0, min(n, j0 + kRescanChunk), redK);
-      if (tid == 0 && m != kEmptyKey) {
-        atomicMin(&s.fnKey[r], m);
-        pairMin(selK, selP, selectionKey(m, r), m);
-      }
-    }
-    blockMinPair(selK, selP, redK, redP);
-    if (tid == 0) {
-      s.selectSlots[2 * g] = selK;
-      s.selectSlots[2 * g + 1] = selP;
-    }
+    lap(1);
+
+    // The previous phase's pending rows, rescanned by all workgroups.
+    if (!rescanned)
+      rescans(listCount, spilled, a);
+    publish(selK, selP, rowAMin, older);
+    prevA = a;
+    prevSize = na + nb;
     lap(3);
     if (!gridBarrier(s.bar, generation, &barrierOk))
       return;
@@ -508,7 +627,6 @@ __global__ void denseSetupKernel(DenseState s, const int *initialSize, const int
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
     s.size[i] = initialSize ? initialSize[i] : 1;
     s.id[i] = initialId ? initialId[i] : i;
-    s.active[i] = 1;
     s.fnKey[i] = kEmptyKey;
   }
 }
@@ -539,9 +657,9 @@ int denseLinkageOnCondensed(float *D, int n, int linkage, const int *initialSize
   if ((n + G - 1) / G > kMaxOwned || G > kThreads)
     return kErrInvalidShape; // beyond any condensed matrix that fits in HBM
   DenseState st{};
-  DevArray<int> size, id, records, rescanRows;
-  DevArray<unsigned char> active;
-  DevArray<unsigned long long> fnKey, selectSlots, rowASlots;
+  DevArray<int> size, id;
+  DevArray<unsigned long long> fnKey, records, fnRescan;
+  DevArray<int> pendingRows;
   DevArray<int2> initItems;
   DevArray<double> tree;
   DevArray<unsigned long long> clocks;
@@ -551,14 +669,25 @@ int denseLinkageOnCondensed(float *D, int n, int linkage, const int *initialSize
   const long long initCount = (long long)(n - 1) * maxChunks;
   const int groups = (G + kGroup - 1) / kGroup;
   const size_t numBarrierWords = (size_t)kLineWords * (3 + groups);
+  // The merge kernel keeps one bit per slot in LDS beside its static arrays.
+  const size_t activeBytes = (size_t)(n + 31) / 32 * sizeof(unsigned int);
+  int device = 0, maxLds = 0;
+  s((int)hipGetDevice(&device));
+  s((int)hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+  if (!s.ok())
+    return s.code;
+  hipFuncAttributes attr{};
+  s((int)hipFuncGetAttributes(&attr, (const void *)denseMergeKernel));
+  if (!s.ok())
+    return s.code;
+  if (activeBytes + attr.sharedSizeBytes > (size_t)maxLds)
+    return kErrInvalidShape; // beyond any condensed matrix that fits in HBM
   s(size.alloc(n));
   s(id.alloc(n));
-  s(active.alloc(n));
   s(fnKey.alloc(n));
-  s(selectSlots.alloc(2 * (size_t)G));
-  s(rowASlots.alloc(G));
-  s(records.alloc((size_t)G * kRecordInts));
-  s(rescanRows.alloc((size_t)G * kMaxOwned));
+  s(records.alloc((size_t)2 * G * kRecordWords));
+  s(fnRescan.alloc(n));
+  s(pendingRows.alloc((size_t)2 * G * kMaxOwned));
   s(initItems.alloc(initCount));
   s(tree.alloc((size_t)(n - 1) * 4));
   s(barrierWords.alloc(numBarrierWords));
@@ -570,12 +699,10 @@ int denseLinkageOnCondensed(float *D, int n, int linkage, const int *initialSize
   st.firstNewId = firstNewId;
   st.size = size;
   st.id = id;
-  st.active = active;
   st.fnKey = fnKey;
-  st.selectSlots = selectSlots;
-  st.rowASlots = rowASlots;
   st.records = records;
-  st.rescanRows = rescanRows;
+  st.fnRescan = fnRescan;
+  st.pendingRows = pendingRows;
   st.items = initItems;
   st.tree = tree;
   st.clocks = nullptr;
@@ -599,7 +726,7 @@ int denseLinkageOnCondensed(float *D, int n, int linkage, const int *initialSize
   if (s.ok()) {
     void *args[] = {&st};
     s((int)hipLaunchCooperativeKernel((const void *)denseMergeKernel, dim3(G), dim3(kThreads),
-                                      args, 0, stream()));
+                                      args, (unsigned int)activeBytes, stream()));
   }
   s(sync());
   unsigned int aborted = 0;
