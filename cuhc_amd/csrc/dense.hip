@@ -178,30 +178,6 @@ struct DenseState {
   Barrier bar;
 };
 
-// Exclusive prefix sum over the workgroup; total receives the sum.
-DEV int blockExclusiveScan(int v, int *waveSums, int &total) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int x = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int y = __shfl_up(x, d, 64);
-    if (lane >= d)
-      x += y;
-  }
-  __syncthreads();
-  if (lane == 63)
-    waveSums[wave] = x;
-  __syncthreads();
-  int before = 0, sum = 0;
-  for (int w = 0; w < kThreads / 64; ++w) {
-    const int sw = waveSums[w];
-    before += w < wave ? sw : 0;
-    sum += sw;
-  }
-  total = sum;
-  return before + x - v;
-}
-
 DEV unsigned long long blockMinKey(unsigned long long v, unsigned long long *red) {
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1)
@@ -242,6 +218,90 @@ DEV void blockMinPair(unsigned long long &k, unsigned long long &p, unsigned lon
   p = redP[0];
   for (int w = 1; w < kThreads / 64; ++w)
     pairMin(k, p, redK[w], redP[w]);
+}
+
+// Workgroup reduction of (k, p) by lexicographic minimum and of x and y by
+// minimum, in one LDS exchange (red: 4 kThreads / 64 words).
+DEV void blockReduce4(unsigned long long &k, unsigned long long &p, unsigned long long &x,
+                      unsigned long long &y, unsigned long long *red) {
+  constexpr int W = kThreads / 64;
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    pairMin(k, p, shflXorKey(k, m), shflXorKey(p, m));
+    x = minKey(x, shflXorKey(x, m));
+    y = minKey(y, shflXorKey(y, m));
+  }
+  const int wave = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    red[wave] = k;
+    red[W + wave] = p;
+    red[2 * W + wave] = x;
+    red[3 * W + wave] = y;
+  }
+  __syncthreads();
+  k = red[0];
+  p = red[W];
+  x = red[2 * W];
+  y = red[3 * W];
+  for (int w = 1; w < W; ++w) {
+    pairMin(k, p, red[w], red[W + w]);
+    x = minKey(x, red[2 * W + w]);
+    y = minKey(y, red[3 * W + w]);
+  }
+}
+
+// blockReduce4 plus, in the same exchange, the exclusive prefix sum of cnt
+// (start; total receives the sum) and whether any cnt exceeds kInlineRows.
+DEV void blockReduce4Scan(unsigned long long &k, unsigned long long &p, unsigned long long &x,
+                          unsigned long long &y, int cnt, int &start, int &total,
+                          bool &spilled, unsigned long long *red, int *redInt) {
+  constexpr int W = kThreads / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    pairMin(k, p, shflXorKey(k, m), shflXorKey(p, m));
+    x = minKey(x, shflXorKey(x, m));
+    y = minKey(y, shflXorKey(y, m));
+  }
+  int incl = cnt;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int v = __shfl_up(incl, d, 64);
+    if (lane >= d)
+      incl += v;
+  }
+  const bool spill = __ballot(cnt > kInlineRows) != 0;
+  __syncthreads();
+  if (lane == 63)
+    redInt[wave] = incl;
+  if (lane == 0) {
+    red[wave] = k;
+    red[W + wave] = p;
+    red[2 * W + wave] = x;
+    red[3 * W + wave] = y;
+    redInt[W + wave] = spill;
+  }
+  __syncthreads();
+  k = red[0];
+  p = red[W];
+  x = red[2 * W];
+  y = red[3 * W];
+  int before = 0, sum = 0;
+  bool any = false;
+  for (int w = 0; w < W; ++w) {
+    if (w > 0) {
+      pairMin(k, p, red[w], red[W + w]);
+      x = minKey(x, red[2 * W + w]);
+      y = minKey(y, red[3 * W + w]);
+    }
+    before += w < wave ? redInt[w] : 0;
+    sum += redInt[w];
+    any |= redInt[W + w] != 0;
+  }
+  start = before + incl - cnt;
+  total = sum;
+  spilled = any;
 }
 
 // Minimum over j in [j0, j1) of denseKey(D(row, j), j), by the whole
@@ -314,8 +374,9 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
   // workgroup w's at [listStart[w], listStart[w + 1]).
   __shared__ int listStart[kThreads + 1];
   __shared__ int listRows[kInlineRows * kThreads];
-  __shared__ int waveSums[kThreads / 64];
-  __shared__ unsigned long long redK[kThreads / 64], redP[kThreads / 64], redA[kThreads / 64];
+  __shared__ unsigned long long redK[kThreads / 64], redP[kThreads / 64];
+  __shared__ unsigned long long red4[4 * (kThreads / 64)];
+  __shared__ int redInt[2 * (kThreads / 64)];
   __shared__ int barrierOk;
   const int n = s.n;
   const int G = gridDim.x, g = blockIdx.x;
@@ -348,9 +409,7 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
       for (int i = tid; i < pendCount[l]; i += kThreads)
         if (isActive(activeBits, pendRow[l][i]))
           pmin = minKey(pmin, pendKey[l][i]);
-    pmin = blockMinKey(pmin, redA);
-    rowA = blockMinKey(rowA, redK);
-    blockMinPair(selK, selP, redK, redP);
+    blockReduce4(selK, selP, rowA, pmin, red4);
     const int fresh = older ^ 1, count = pendCount[fresh];
     if (count > kInlineRows)
       for (int i = tid; i < count; i += kThreads)
@@ -391,9 +450,9 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
   // Rescans of the listed rows, shared by all workgroups in column chunks:
   // item t is chunk t % maxChunks of list row t / maxChunks. Column skip
   // (the merged slot, whose entries their owners are rewriting) is left out.
-  auto rescans = [&](int count, bool spilled, int skip) {
+  auto rescans = [&](int count, bool spilled, int skip, long long first) {
     const long long numItems = (long long)count * maxChunks;
-    for (long long t = g; t < numItems; t += G) {
+    for (long long t = first; t < numItems; t += G) {
       const int r = listRow((int)(t / maxChunks), spilled);
       const int j0 = r + 1 + (int)(t % maxChunks) * kRescanChunk;
       if (j0 >= n || !isActive(activeBits, r))
@@ -449,32 +508,42 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
 
   int prevA = -1, prevSize = 0; // the previous merge's slot a and merged size
   for (int step = 0; step < n - 1; ++step) {
-    // Records of the previous phase: every workgroup reduces them the same
-    // way. The previous merged row's new fn is the minimum of the parts.
+    // This phase's inputs are loaded together: the latest records, and the
+    // rescan results of this workgroup's rows rescanned in the previous phase.
     unsigned long long k = kEmptyKey, p = kEmptyKey, part = kEmptyKey, pmin = kEmptyKey;
     int cnt = 0;
-    int inl[kInlineRows];
+    int inl[kInlineRows] = {};
     if (tid < G) {
       const unsigned long long *r = recordsOf((published - 1) & 1) + (size_t)tid * kRecordWords;
-      k = r[0];
-      p = r[1];
-      part = r[2];
-      pmin = r[3];
-      cnt = (int)r[4];
-      for (int w = 0; w < kInlineRows / 2; ++w) {
-        const unsigned long long x = r[5 + w];
-        inl[2 * w] = (int)(unsigned)x;
-        inl[2 * w + 1] = (int)(unsigned)(x >> 32);
+      unsigned long long w[kRecordWords];
+#pragma unroll
+      for (int i = 0; i < kRecordWords; ++i)
+        w[i] = r[i];
+      k = w[0];
+      p = w[1];
+      part = w[2];
+      pmin = w[3];
+      cnt = (int)w[4];
+#pragma unroll
+      for (int i = 0; i < kInlineRows / 2; ++i) {
+        inl[2 * i] = (int)(unsigned)w[5 + i];
+        inl[2 * i + 1] = (int)(unsigned)(w[5 + i] >> 32);
       }
     }
-    part = blockMinKey(part, redA);
-    pmin = blockMinKey(pmin, redK);
-    blockMinPair(k, p, redK, redP);
+    const int numInstall = pendCount[older];
+    int installRow = -1;
+    unsigned long long installFn = kEmptyKey;
+    if (tid < numInstall) {
+      installRow = pendRow[older][tid];
+      installFn = s.fnRescan[installRow];
+    }
+    // Every workgroup reduces the records the same way. The previous merged
+    // row's new fn is the minimum of the parts.
+    int listCount = 0, start = 0;
+    bool spilled = false;
+    blockReduce4Scan(k, p, part, pmin, cnt, start, listCount, spilled, red4, redInt);
     if (prevA >= 0)
       pairMin(k, p, selectionKey(part, prevA), part);
-    int listCount = 0;
-    const int start = blockExclusiveScan(cnt, waveSums, listCount);
-    const bool spilled = __syncthreads_or(cnt > kInlineRows);
     if (tid < G) {
       listStart[tid] = start;
       if (!spilled)
@@ -483,6 +552,7 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
     }
     if (tid == 0) {
       listStart[G] = listCount;
+      pendCount[older] = 0;
       if (prevA >= mine0 && prevA < mine1) {
         ownFn[prevA - mine0] = part;
         ownSize[prevA - mine0] = prevSize;
@@ -491,14 +561,23 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
       }
     }
     // Rows rescanned in the previous phase settle; this phase's list is new.
-    install(older);
+    if (installRow >= 0) {
+      ownPending[installRow - mine0] = 0;
+      ownFn[installRow - mine0] = isActive(activeBits, installRow) ? installFn : kEmptyKey;
+    }
+    for (int i = kThreads + tid; i < numInstall; i += kThreads) {
+      const int r = pendRow[older][i];
+      ownPending[r - mine0] = 0;
+      ownFn[r - mine0] = isActive(activeBits, r) ? s.fnRescan[r] : kEmptyKey;
+    }
+    __syncthreads();
     older ^= 1;
     bool rescanned = false;
     if (!(k < pmin)) {
       // A pending row could hold the lowest key: rescan the previous phase's
       // rows now, settle every pending row, and select again from records
       // that hold no pending row (two extra barriers; rare).
-      rescans(listCount, spilled, -1);
+      rescans(listCount, spilled, -1, g);
       if (!gridBarrier(s.bar, generation, &barrierOk))
         return;
       install(older);
@@ -530,21 +609,37 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
     }
     const float dab = fromOrderedBits((unsigned)(p >> 32));
     // Sizes and ids in memory are at least one phase old; the previous
-    // merge's cluster is known here.
+    // merge's cluster is known here. Their loads join the update's loads.
     const int na = a == prevA ? prevSize : s.size[a];
     const int nb = b == prevA ? prevSize : s.size[b];
+    int ia = 0, ib = 0;
     if (g == 0 && tid == 0) {
       const int prevId = s.firstNewId + step - 1;
-      const int ia = a == prevA ? prevId : s.id[a], ib = b == prevA ? prevId : s.id[b];
-      double *row = s.tree + (size_t)step * 4;
-      row[0] = (double)min(ia, ib);
-      row[1] = (double)max(ia, ib);
-      row[2] = (double)dab;
-      row[3] = (double)(na + nb);
+      ia = a == prevA ? prevId : s.id[a];
+      ib = b == prevA ? prevId : s.id[b];
     }
+    // The update skips b itself, so its bit is cleared without a barrier
+    // here (the next one, after the update, publishes it to the rescans).
     if (tid == 0)
       activeBits[b >> 5] &= ~(1u << (b & 31));
-    __syncthreads();
+    // The first rescan item of this workgroup: its loads are issued before
+    // the update (no entry it reads is written in this phase but D(row, a),
+    // which the scan leaves out).
+    const long long numItems = rescanned ? 0 : (long long)listCount * maxChunks;
+    int firstRow = -1, firstJ0 = n;
+    float firstV[kRescanPer];
+    if (g < numItems) {
+      firstRow = listRow(g / maxChunks, spilled);
+      firstJ0 = firstRow + 1 + (g % maxChunks) * kRescanChunk;
+      if (firstJ0 < n) {
+        const size_t base = condensedIndex(firstRow, firstRow + 1, n) - (size_t)(firstRow + 1);
+#pragma unroll
+        for (int u = 0; u < kRescanPer; ++u) {
+          const int j = firstJ0 + tid + u * kThreads;
+          firstV[u] = j < n && j < firstJ0 + kRescanChunk ? s.D[base + j] : 0.0f;
+        }
+      }
+    }
     lap(0);
 
     // Lance-Williams update of D(a, k) for this workgroup's k, its settled
@@ -604,12 +699,35 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
     }
     if (tid == 0 && b >= mine0 && b < mine1)
       ownFn[b - mine0] = kEmptyKey;
+    if (g == 0 && tid == 0) {
+      double *row = s.tree + (size_t)step * 4;
+      row[0] = (double)min(ia, ib);
+      row[1] = (double)max(ia, ib);
+      row[2] = (double)dab;
+      row[3] = (double)(na + nb);
+    }
     __syncthreads();
     lap(1);
 
-    // The previous phase's pending rows, rescanned by all workgroups.
+    // The first rescan item, loaded before the update (the same item for
+    // every thread of the workgroup).
+    if (g < numItems && firstJ0 < n && isActive(activeBits, firstRow)) {
+      const int j1 = min(n, firstJ0 + kRescanChunk);
+      unsigned long long best = kEmptyKey;
+#pragma unroll
+      for (int u = 0; u < kRescanPer; ++u) {
+        const int j = firstJ0 + tid + u * kThreads;
+        if (j < j1 && j != a && isActive(activeBits, j))
+          best = minKey(best, denseKey(firstV[u], (unsigned)j));
+      }
+      best = blockMinKey(best, redK);
+      if (tid == 0 && best != kEmptyKey)
+        atomicMin(&s.fnRescan[firstRow], best);
+    }
+    // The rest of the previous phase's pending rows, rescanned by all
+    // workgroups.
     if (!rescanned)
-      rescans(listCount, spilled, a);
+      rescans(listCount, spilled, a, (long long)g + G);
     publish(selK, selP, rowAMin, older);
     prevA = a;
     prevSize = na + nb;
