@@ -51,6 +51,7 @@ constexpr int kChunk = 8192;                     // columns per initial-scan wor
 constexpr int kRescanChunk = 4096;               // columns per rescan work item
 constexpr int kRescanPer = kRescanChunk / kThreads;
 constexpr int kMaxOwned = 2048;                  // rows per workgroup (LDS row state)
+constexpr int kUpdatePer = 2;                    // owned rows per thread per load batch
 constexpr int kRecordWords = 8;                  // u64 words per workgroup record
 constexpr int kInlineRows = 6;                   // pending rows carried in a record
 constexpr int kGroup = 16;                       // workgroups per first-level barrier counter
@@ -220,16 +221,17 @@ DEV void blockMinPair(unsigned long long &k, unsigned long long &p, unsigned lon
     pairMin(k, p, redK[w], redP[w]);
 }
 
-// Workgroup reduction of (k, p) by lexicographic minimum and of x and y by
-// minimum, in one LDS exchange (red: 4 kThreads / 64 words).
-DEV void blockReduce4(unsigned long long &k, unsigned long long &p, unsigned long long &x,
-                      unsigned long long &y, unsigned long long *red) {
+// Workgroup reduction of (k, p) by lexicographic minimum and of x, y and z
+// by minimum, in one LDS exchange (red: 5 kThreads / 64 words).
+DEV void blockReduce5(unsigned long long &k, unsigned long long &p, unsigned long long &x,
+                      unsigned long long &y, unsigned long long &z, unsigned long long *red) {
   constexpr int W = kThreads / 64;
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) {
     pairMin(k, p, shflXorKey(k, m), shflXorKey(p, m));
     x = minKey(x, shflXorKey(x, m));
     y = minKey(y, shflXorKey(y, m));
+    z = minKey(z, shflXorKey(z, m));
   }
   const int wave = threadIdx.x >> 6;
   __syncthreads();
@@ -238,20 +240,23 @@ DEV void blockReduce4(unsigned long long &k, unsigned long long &p, unsigned lon
     red[W + wave] = p;
     red[2 * W + wave] = x;
     red[3 * W + wave] = y;
+    red[4 * W + wave] = z;
   }
   __syncthreads();
   k = red[0];
   p = red[W];
   x = red[2 * W];
   y = red[3 * W];
+  z = red[4 * W];
   for (int w = 1; w < W; ++w) {
     pairMin(k, p, red[w], red[W + w]);
     x = minKey(x, red[2 * W + w]);
     y = minKey(y, red[3 * W + w]);
+    z = minKey(z, red[4 * W + w]);
   }
 }
 
-// blockReduce4 plus, in the same exchange, the exclusive prefix sum of cnt
+// A blockReduce5 without z plus, in the same exchange, the exclusive prefix sum of cnt
 // (start; total receives the sum) and whether any cnt exceeds kInlineRows.
 DEV void blockReduce4Scan(unsigned long long &k, unsigned long long &p, unsigned long long &x,
                           unsigned long long &y, int cnt, int &start, int &total,
@@ -375,7 +380,7 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
   __shared__ int listStart[kThreads + 1];
   __shared__ int listRows[kInlineRows * kThreads];
   __shared__ unsigned long long redK[kThreads / 64], redP[kThreads / 64];
-  __shared__ unsigned long long red4[4 * (kThreads / 64)];
+  __shared__ unsigned long long red4[5 * (kThreads / 64)];
   __shared__ int redInt[2 * (kThreads / 64)];
   __shared__ int barrierOk;
   const int n = s.n;
@@ -399,9 +404,12 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
   auto recordsOf = [&](int buffer) { return s.records + (size_t)buffer * G * kRecordWords; };
   auto pendingOf = [&](int buffer) { return s.pendingRows + (size_t)buffer * G * kMaxOwned; };
   // Publishes this workgroup's record: settled rows, row-a part, pending
-  // lower bound, and the rows made pending in this phase (list[older ^ 1]).
+  // lower bound, and the rows made pending in this phase (list[older ^ 1]);
+  // folds this thread's part of a rescan chunk of row chunkRow (-1: none) into
+  // that row's result in the same reduction.
   auto publish = [&](unsigned long long selK, unsigned long long selP,
-                     unsigned long long rowA, int older) {
+                     unsigned long long rowA, int older, unsigned long long chunkMin,
+                     int chunkRow) {
     const int buffer = published & 1;
     unsigned long long *record = recordsOf(buffer) + (size_t)g * kRecordWords;
     unsigned long long pmin = kEmptyKey;
@@ -409,7 +417,9 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
       for (int i = tid; i < pendCount[l]; i += kThreads)
         if (isActive(activeBits, pendRow[l][i]))
           pmin = minKey(pmin, pendKey[l][i]);
-    blockReduce4(selK, selP, rowA, pmin, red4);
+    blockReduce5(selK, selP, rowA, pmin, chunkMin, red4);
+    if (tid == 0 && chunkRow >= 0 && chunkMin != kEmptyKey)
+      atomicMin(&s.fnRescan[chunkRow], chunkMin);
     const int fresh = older ^ 1, count = pendCount[fresh];
     if (count > kInlineRows)
       for (int i = tid; i < count; i += kThreads)
@@ -499,7 +509,7 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
   {
     unsigned long long k = kEmptyKey, p = kEmptyKey;
     settledMin(k, p, -1);
-    publish(k, p, kEmptyKey, older);
+    publish(k, p, kEmptyKey, older, kEmptyKey, -1);
   }
   if (!gridBarrier(s.bar, generation, &barrierOk))
     return;
@@ -583,7 +593,7 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
       install(older);
       unsigned long long k2 = kEmptyKey, p2 = kEmptyKey;
       settledMin(k2, p2, -1);
-      publish(k2, p2, kEmptyKey, older);
+      publish(k2, p2, kEmptyKey, older, kEmptyKey, -1);
       if (!gridBarrier(s.bar, generation, &barrierOk))
         return;
       k = kEmptyKey;
@@ -646,56 +656,75 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
     // rows' fn, and its part of row a's new fn.
     unsigned long long rowAMin = kEmptyKey, selK = kEmptyKey, selP = kEmptyKey;
     const int fresh = older ^ 1;
-    for (int kk = mine0 + tid; kk < mine1; kk += kThreads) {
-      if (kk == a || kk == b || !isActive(activeBits, kk))
-        continue;
-      const size_t iak = kk < a ? condensedIndex(kk, a, n) : condensedIndex(a, kk, n);
-      const size_t ibk = kk < b ? condensedIndex(kk, b, n) : condensedIndex(b, kk, n);
-      const float dak = s.D[iak], dbk = s.D[ibk];
-      const unsigned long long f = ownFn[kk - mine0];
-      const float nd =
-          (float)lanceWilliams(s.linkage, dak, dbk, dab, na, nb, ownSize[kk - mine0]);
-      s.D[iak] = nd;
-      if (kk > a)
-        rowAMin = minKey(rowAMin, denseKey(nd, (unsigned)kk));
-      if (ownPending[kk - mine0]) {
-        // Being rescanned without column a: its new D(kk, a) joins here.
-        if (kk < a)
-          atomicMin(&s.fnRescan[kk], denseKey(nd, (unsigned)a));
-        continue;
-      }
-      unsigned long long nf = f;
-      bool rescan = false;
-      if (kk < a) {
-        // Every entry of row kk but D(kk, a) and D(kk, b) is unchanged.
-        const int fk = f == kEmptyKey ? -1 : keyIndex(f);
-        const unsigned long long cand = denseKey(nd, (unsigned)a);
-        if (fk == a || fk == b) {
-          // The other entries were >= the old minimum, and any equal one has
-          // a higher column than b: (nd, a) is the minimum if nd is no larger.
-          if (orderedBits(nd) <= (unsigned)(f >> 32))
-            nf = cand;
-          else
-            rescan = true;
-        } else if (cand < f) {
-          nf = cand;
+    // Rows in batches of kUpdatePer per thread, every load of a batch issued
+    // before the first is used: one memory round trip per batch.
+    for (int batch = mine0; batch < mine1; batch += kUpdatePer * kThreads) {
+      bool live[kUpdatePer];
+      size_t iakOf[kUpdatePer];
+      float dakOf[kUpdatePer], dbkOf[kUpdatePer];
+#pragma unroll
+      for (int u = 0; u < kUpdatePer; ++u) {
+        const int kk = batch + tid + u * kThreads;
+        live[u] = kk < mine1 && kk != a && kk != b && isActive(activeBits, kk);
+        if (live[u]) {
+          iakOf[u] = kk < a ? condensedIndex(kk, a, n) : condensedIndex(a, kk, n);
+          const size_t ibk = kk < b ? condensedIndex(kk, b, n) : condensedIndex(b, kk, n);
+          dakOf[u] = s.D[iakOf[u]];
+          dbkOf[u] = s.D[ibk];
         }
-      } else if (kk < b && f != kEmptyKey && keyIndex(f) == b) {
-        rescan = true; // its nearest retired
       }
-      if (rescan) {
-        // Pending until the next phase's rescan; its new key is at least
-        // the old one (every linkage here is reducible).
-        const int r = atomicAdd(&pendCount[fresh], 1);
-        pendRow[fresh][r] = kk;
-        pendKey[fresh][r] = selectionKey(f, kk);
-        ownPending[kk - mine0] = 1;
-        s.fnRescan[kk] = kEmptyKey;
-        nf = kEmptyKey;
-      } else {
-        pairMin(selK, selP, selectionKey(nf, kk), nf);
+#pragma unroll
+      for (int u = 0; u < kUpdatePer; ++u) {
+        if (!live[u])
+          continue;
+        const int kk = batch + tid + u * kThreads;
+        const size_t iak = iakOf[u];
+        const float dak = dakOf[u], dbk = dbkOf[u];
+        const unsigned long long f = ownFn[kk - mine0];
+        const float nd =
+            (float)lanceWilliams(s.linkage, dak, dbk, dab, na, nb, ownSize[kk - mine0]);
+        s.D[iak] = nd;
+        if (kk > a)
+          rowAMin = minKey(rowAMin, denseKey(nd, (unsigned)kk));
+        if (ownPending[kk - mine0]) {
+          // Being rescanned without column a: its new D(kk, a) joins here.
+          if (kk < a)
+            atomicMin(&s.fnRescan[kk], denseKey(nd, (unsigned)a));
+          continue;
+        }
+        unsigned long long nf = f;
+        bool rescan = false;
+        if (kk < a) {
+          // Every entry of row kk but D(kk, a) and D(kk, b) is unchanged.
+          const int fk = f == kEmptyKey ? -1 : keyIndex(f);
+          const unsigned long long cand = denseKey(nd, (unsigned)a);
+          if (fk == a || fk == b) {
+            // The other entries were >= the old minimum, and any equal one has
+            // a higher column than b: (nd, a) is the minimum if nd is no larger.
+            if (orderedBits(nd) <= (unsigned)(f >> 32))
+              nf = cand;
+            else
+              rescan = true;
+          } else if (cand < f) {
+            nf = cand;
+          }
+        } else if (kk < b && f != kEmptyKey && keyIndex(f) == b) {
+          rescan = true; // its nearest retired
+        }
+        if (rescan) {
+          // Pending until the next phase's rescan; its new key is at least
+          // the old one (every linkage here is reducible).
+          const int r = atomicAdd(&pendCount[fresh], 1);
+          pendRow[fresh][r] = kk;
+          pendKey[fresh][r] = selectionKey(f, kk);
+          ownPending[kk - mine0] = 1;
+          s.fnRescan[kk] = kEmptyKey;
+          nf = kEmptyKey;
+        } else {
+          pairMin(selK, selP, selectionKey(nf, kk), nf);
+        }
+        ownFn[kk - mine0] = nf;
       }
-      ownFn[kk - mine0] = nf;
     }
     if (tid == 0 && b >= mine0 && b < mine1)
       ownFn[b - mine0] = kEmptyKey;
@@ -710,25 +739,24 @@ __global__ void __launch_bounds__(kThreads) denseMergeKernel(DenseState s) {
     lap(1);
 
     // The first rescan item, loaded before the update (the same item for
-    // every thread of the workgroup).
-    if (g < numItems && firstJ0 < n && isActive(activeBits, firstRow)) {
+    // every thread of the workgroup); its minimum joins the record's
+    // reduction.
+    unsigned long long firstMin = kEmptyKey;
+    const bool firstLive = g < numItems && firstJ0 < n && isActive(activeBits, firstRow);
+    if (firstLive) {
       const int j1 = min(n, firstJ0 + kRescanChunk);
-      unsigned long long best = kEmptyKey;
 #pragma unroll
       for (int u = 0; u < kRescanPer; ++u) {
         const int j = firstJ0 + tid + u * kThreads;
         if (j < j1 && j != a && isActive(activeBits, j))
-          best = minKey(best, denseKey(firstV[u], (unsigned)j));
+          firstMin = minKey(firstMin, denseKey(firstV[u], (unsigned)j));
       }
-      best = blockMinKey(best, redK);
-      if (tid == 0 && best != kEmptyKey)
-        atomicMin(&s.fnRescan[firstRow], best);
     }
     // The rest of the previous phase's pending rows, rescanned by all
     // workgroups.
     if (!rescanned)
       rescans(listCount, spilled, a, (long long)g + G);
-    publish(selK, selP, rowAMin, older);
+    publish(selK, selP, rowAMin, older, firstMin, firstLive ? firstRow : -1);
     prevA = a;
     prevSize = na + nb;
     lap(3);
