@@ -87,7 +87,7 @@ _declare("cuhc_mlp_train", _int, _f32p, _f32p, _f32p, _int, _int, _float)
 PRUNE_MODES = {"auto": 0, "always": 1, "never": 2}
 PRECISION_MODES = {"auto": 0, "fp32": 1, "mixed": 2}
 PHASES = ["distance_tiles", "mfma_candidates", "recheck", "core", "contract", "dense", "rnn",
-          "mlp", "dense_select", "dense_update", "dense_rescan", "dense_barrier"]
+          "mlp", "dense_select", "dense_update", "dense_rescan", "dense_barrier", "host_tree"]
 
 # The library's status for NaN or infinite input (reference src/mst.cuh:80-81).
 NON_FINITE = -6

@@ -8,6 +8,7 @@
 #include "primitives.hpp"
 
 #include <algorithm>
+#include <chrono>
 
 namespace cuhc {
 
@@ -84,20 +85,17 @@ int treeFromDeviceEdges(const DeviceEdges &edges, int n, double *tree) {
   hipLaunchKernelGGL(edgeGatherKernel, grid, dim3(256), 0, stream(), edges.u.get(),
                      edges.v.get(), edges.w.get(), idxOut.get(), m, packed.get());
   s((int)hipGetLastError());
-  // The tree is written straight from the staged edges: runs of equal
-  // (weight, u) are ordered by v in place first.
+  // The tree is written straight from the staged edges; runs of equal
+  // (weight, u) are ordered by v in place as the assembly reaches them.
   s(downloadStaged(packed, m * sizeof(PackedEdge), [&](const void *p) {
+    const auto t0 = std::chrono::steady_clock::now();
     PackedEdge *hp = static_cast<PackedEdge *>(const_cast<void *>(p));
-    for (int i = 0; i < m;) {
-      int j = i + 1;
-      while (j < m && hp[j].w == hp[i].w && hp[j].u == hp[i].u)
-        ++j;
-      if (j - i > 1)
-        std::sort(hp + i, hp + j,
-                  [](const PackedEdge &x, const PackedEdge &y) { return x.v < y.v; });
-      i = j;
-    }
     treeFromOrderedEdges(hp, (size_t)m, n, tree);
+    if (timingEnabled())
+      addMs(kPhaseHostTree,
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                .count(),
+            1);
   }));
   return s.code;
 }

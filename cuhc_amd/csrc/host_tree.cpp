@@ -73,40 +73,79 @@ bool mergesToTreeOrder(std::vector<Edge> &merges, int n) {
 
 namespace {
 
-template <class E> void assembleTree(const E *merges, size_t m, int n, double *tree) {
-  Clusters sets(n);
-  Node *node = sets.node;
-  // The finds start at random points: the records of the endpoints 16 merges
-  // ahead, and their links 8 merges ahead, are prefetched.
+// Union-find for the ordered merges: a parent link per point (-1 at a root)
+// and, per root, its cluster's size and scipy id in one 8-byte record, both
+// kept between calls: 4 bytes per point touched at random plus one record per
+// root, instead of a 24-byte node, halves the assembly time at 10^6 points.
+// Finds halve the path as they climb; the links of the endpoints 16 merges
+// ahead are prefetched. With ORDER_TIES, a run of merges with equal (weight,
+// u) is sorted by v when the loop reaches it (the run is read anyway).
+struct RootInfo {
+  int size, id;
+};
+
+template <bool ORDER_TIES, class E> void assembleTree(E *merges, size_t m, int n, double *tree) {
+  static thread_local std::vector<int> link;
+  static thread_local std::vector<RootInfo> root;
+  if (link.size() < (size_t)n) {
+    link.resize(n);
+    root.resize(n);
+  }
+  for (int i = 0; i < n; ++i) {
+    link[i] = -1;
+    root[i] = {1, i};
+  }
+  int *up = link.data();
+  RootInfo *info = root.data();
+  auto find = [up](int x) {
+    while (up[x] >= 0) {
+      const int p = up[x];
+      if (up[p] >= 0)
+        up[x] = up[p];
+      x = p;
+    }
+    return x;
+  };
   constexpr size_t kAhead = 16;
+  size_t runEnd = 0; // merges before runEnd have their ties ordered
   for (size_t s = 0; s < m; ++s) {
+    if constexpr (ORDER_TIES) {
+      if (s >= runEnd) {
+        size_t j = s + 1;
+        while (j < m && merges[j].w == merges[s].w && merges[j].u == merges[s].u)
+          ++j;
+        if (j - s > 1)
+          std::sort(merges + s, merges + j, [](const E &x, const E &y) { return x.v < y.v; });
+        runEnd = j;
+      }
+    }
     if (s + kAhead < m) {
-      __builtin_prefetch(&node[merges[s + kAhead].u], 1);
-      __builtin_prefetch(&node[merges[s + kAhead].v], 1);
+      __builtin_prefetch(&up[merges[s + kAhead].u], 1);
+      __builtin_prefetch(&up[merges[s + kAhead].v], 1);
     }
-    if (s + kAhead / 2 < m) {
-      __builtin_prefetch(&node[node[merges[s + kAhead / 2].u].link], 1);
-      __builtin_prefetch(&node[node[merges[s + kAhead / 2].v].link], 1);
-    }
-    const int ra = sets.find(merges[s].u), rb = sets.find(merges[s].v);
-    const int a = node[ra].id, b = node[rb].id;
+    int ra = find(merges[s].u), rb = find(merges[s].v);
+    const RootInfo ia = info[ra], ib = info[rb];
+    const int total = ia.size + ib.size;
     double *row = tree + s * 4;
-    row[0] = std::min(a, b);
-    row[1] = std::max(a, b);
+    row[0] = std::min(ia.id, ib.id);
+    row[1] = std::max(ia.id, ib.id);
     row[2] = (double)merges[s].w;
-    row[3] = node[ra].size + node[rb].size;
-    node[sets.join(ra, rb)].id = n + (int)s;
+    row[3] = total;
+    if (ia.size < ib.size)
+      std::swap(ra, rb);
+    up[rb] = ra;
+    info[ra] = {total, n + (int)s};
   }
 }
 
 } // namespace
 
 void treeFromOrderedMerges(const std::vector<Edge> &merges, int n, double *tree) {
-  assembleTree(merges.data(), merges.size(), n, tree);
+  assembleTree<false>(merges.data(), merges.size(), n, tree);
 }
 
-void treeFromOrderedEdges(const Edge32 *edges, size_t m, int n, double *tree) {
-  assembleTree(edges, m, n, tree);
+void treeFromOrderedEdges(Edge32 *edges, size_t m, int n, double *tree) {
+  assembleTree<true>(edges, m, n, tree);
 }
 
 void sortEdges(std::vector<Edge> &edges) { std::sort(edges.begin(), edges.end(), edgeLess); }

@@ -37,12 +37,14 @@ bool mergesToTreeOrder(std::vector<Edge> &merges, int n);
 // tree order; the cluster made by merge s is n + s.
 void treeFromOrderedMerges(const std::vector<Edge> &merges, int n, double *tree);
 
-// The same from FP32-weighted edges (the layout the device sort produces).
+// The same from FP32-weighted edges (the layout the device sort produces),
+// sorted by (weight, smaller endpoint) only: each run of equal (weight, u) is
+// ordered by v in place as the assembly reaches it.
 struct Edge32 {
   int u, v; // u < v
   float w;
 };
-void treeFromOrderedEdges(const Edge32 *edges, size_t m, int n, double *tree);
+void treeFromOrderedEdges(Edge32 *edges, size_t m, int n, double *tree);
 
 // Sorts by edgeLess on the host.
 void sortEdges(std::vector<Edge> &edges);

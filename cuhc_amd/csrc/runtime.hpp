@@ -122,7 +122,8 @@ enum Phase : int {
   kPhaseDenseUpdate = 9,    // dense loop, device clock: Lance-Williams update
   kPhaseDenseRescan = 10,   // dense loop, device clock: row rescans and selection records
   kPhaseDenseBarrier = 11,  // dense loop, device clock: grid barriers
-  kNumPhases = 12,
+  kPhaseHostTree = 12,      // host clock: tree assembly from the downloaded edges
+  kNumPhases = 13,
 };
 
 // Times the enclosed launches of one phase; a timer nested inside another of

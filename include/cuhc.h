@@ -181,7 +181,8 @@ void cuhc_release(void);
  * Phases: 0 FP32 distance tiles, 1 FP16 MFMA candidates, 2 FP32 recheck,
  * 3 core distances, 4 component bookkeeping, 5 dense loop, 6 RNN kernels,
  * 7 learned-radius network; 8-11 split the dense loop by the device clock
- * into pair selection, Lance-Williams update, row rescans and grid barriers. */
+ * into pair selection, Lance-Williams update, row rescans and grid barriers;
+ * 12 host tree assembly from the downloaded edges (host clock). */
 int cuhc_set_timing(int on);
 /* Fills up to count phases: total ms, launches, algorithmic units (pairs). */
 void cuhc_read_timings(double *ms, long long *launches, long long *units, int count);
