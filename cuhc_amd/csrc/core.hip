@@ -16,6 +16,7 @@
 #include "tile.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace cuhc {
 
@@ -65,8 +66,19 @@ DEV void insertSorted(float *list, int k, float v) {
 // tile tileList[blockIdx.x] with a list) or listed points (rowsX: numRows
 // gathered rows, rowPos their positions in X, for the exact search of
 // selected points).
+// Four waves per SIMD for the one-fill tiles (d <= 8) of the squared-sum
+// metrics: the pass waits on tile loads, and the extra wave hides more of them
+// than its few spills cost (1M points, d = 8: 195 to 168 ms). The other folds
+// would spill far more at 128 VGPRs.
+template <class T, class Metric> constexpr int coreMinWaves() {
+  return T::FILL <= 8 &&
+                 (std::is_same<Metric, Euclidean>::value || std::is_same<Metric, SqEuclidean>::value)
+             ? 4
+             : 1;
+}
+
 template <class T, class Metric>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, (coreMinWaves<T, Metric>()))
     kthNearestFusedKernel(const float *__restrict__ X, int n, int d, Metric metric, int k,
                           float *__restrict__ core, CoreBoxes boxes, int stepLimit,
                           const float *__restrict__ rowsX, int numRows,
