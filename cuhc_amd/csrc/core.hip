@@ -111,6 +111,11 @@ __global__ void __launch_bounds__(256, (coreMinWaves<T, Metric>()))
     largestKth = INFINITY; // every list starts empty
     inserted = 0;
   }
+  // The 8-feature tiles serve d <= 8 only (one fill): the row slice is
+  // staged once for every tile.
+  constexpr bool kRowsResident = T::FILL == 8;
+  if (kRowsResident)
+    stageRows<T>(sm, A, nA, d, row0);
   __syncthreads();
   const float slack = 1.0f - coreBoxSlack(d);
   const int numTiles = (n + T::DIM - 1) / T::DIM;
@@ -156,7 +161,7 @@ __global__ void __launch_bounds__(256, (coreMinWaves<T, Metric>()))
       // (most pairs of a tile are not, which skips their sqrt).
       constexpr bool kFilter = Metric::kAccFilter;
       float v[TR][TR];
-      computeTile<T, Metric, !kFilter>(sm, A, nA, X, n, d, metric, row0, col0, v);
+      computeTile<T, Metric, !kFilter, kRowsResident>(sm, A, nA, X, n, d, metric, row0, col0, v);
 #pragma unroll
       for (int r = 0; r < TR; ++r) {
         const int rit = T::rowInTile(r);

@@ -533,6 +533,11 @@ __global__ void __launch_bounds__(256)
     rowVal[r] = weight.value(row < n ? row : 0);
   }
   unsigned long long computed = 0;
+  // The 8-feature tiles serve d <= 8 only (one fill): the row slice is
+  // staged once for every tile.
+  constexpr bool kRowsResident = T::FILL == 8;
+  if (kRowsResident)
+    stageRows<T>(sm, X, n, d, row0);
   __syncthreads();
 
   // The bound: the largest need over the open rows (inf while a row of a
@@ -608,7 +613,8 @@ __global__ void __launch_bounds__(256)
       // computeTile's barriers; rewritten only after this tile's last barrier).
       const ColumnStage<Weight> stage{comp, orig, weight, col0, n, T::DIM,
                                       sColComp, sColVal, sColIdx};
-      computeTile<T, Metric, !kFilter>(sm, X, n, X, n, d, metric, row0, col0, w, stage);
+      computeTile<T, Metric, !kFilter, kRowsResident>(sm, X, n, X, n, d, metric, row0, col0, w,
+                                                      stage);
       int colComp[TR];
       typename Weight::Value colVal[TR];
       unsigned int colIdx[TR];

@@ -88,8 +88,11 @@ struct NoExtraStage {
 // thread of the workgroup must call it. extra.load() is issued with the first
 // fill's loads and extra.store() runs after that fill's LDS stores, before
 // its barrier: per-tile data the caller stages in LDS shares the fill's
-// memory round trip.
-template <class T, class Metric, bool kFinish = true, class Extra = NoExtraStage>
+// memory round trip. With kRowsResident (only for d <= T::FILL: one fill)
+// sm.a already holds the row slice (stageRows) and only the columns are
+// staged.
+template <class T, class Metric, bool kFinish = true, bool kRowsResident = false,
+          class Extra = NoExtraStage>
 DEV void computeTile(TileSmem<T> &sm, const float *__restrict__ A, int nA,
                      const float *__restrict__ B, int nB, int d, const Metric &metric, int rowA0,
                      int colB0, float (&out)[T::TR][T::TR], const Extra &extra = Extra{}) {
@@ -110,7 +113,8 @@ DEV void computeTile(TileSmem<T> &sm, const float *__restrict__ A, int nA,
     typename Extra::Value ev{};
     if (f0 == 0)
       ev = extra.load();
-    stageSlice<T>(sm.a, A, nA, d, rowA0, f0, nf8);
+    if (!kRowsResident)
+      stageSlice<T>(sm.a, A, nA, d, rowA0, f0, nf8);
     stageSlice<T>(sm.b, B, nB, d, colB0, f0, nf8);
     if (f0 == 0)
       extra.store(ev);
@@ -145,6 +149,16 @@ DEV void computeTile(TileSmem<T> &sm, const float *__restrict__ A, int nA,
       else
         out[r][c] = acc[r][c];
     }
+}
+
+// Stages rows [rowA0, rowA0 + DIM) of A into sm.a once, for the
+// kRowsResident tiles of a workgroup whose rows stay fixed (d <= T::FILL).
+// Contains __syncthreads().
+template <class T>
+DEV void stageRows(TileSmem<T> &sm, const float *__restrict__ A, int nA, int d, int rowA0) {
+  __syncthreads();
+  stageSlice<T>(sm.a, A, nA, d, rowA0, 0, (d + 7) & ~7);
+  __syncthreads();
 }
 
 // Tile geometry per metric: wide (cosine) accumulators get 4x4 outputs per
