@@ -70,6 +70,7 @@ constexpr int kWaveRows = 128;     // 4 row blocks of 32
 constexpr int kBlockRows = 4 * kWaveRows;
 constexpr int kCandCap = 64; // list entries per (row, half)
 constexpr int kTopK = 4; // nearest points kept per row for the next round
+constexpr int kDryTilesFirstRound = 16;
 
 std::atomic<int> gLayoutChecked{0}; // 0 unknown, 1 ok, -1 mismatch
 unsigned long long gStats[4];
@@ -230,6 +231,7 @@ struct CandArgs {
   unsigned int *rowMin; // numRows, thetaBits of the row minimum shared by column splits
   unsigned long long *recorded; // [0] entries recorded, [4] row-column pairs computed
   int rank; // theta from the rank-th smallest half-block minimum (1 or kTopK)
+  int dryTiles; // tiles of the dry run before recording starts (at least 1)
 };
 
 // Order-preserving 32-bit image of a float (any sign), for atomicMin across
@@ -513,29 +515,36 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
     const int j = it0 + i;
     return skipped && j >= tA ? j + skipped : j;
   };
-  if (iters > 0) {
-    request(tileOf(0), 0);
-    if (iters > 1)
-      request(tileOf(1), 1);
+  // Steps [0, dryTiles) are a dry run over the first dryTiles tiles: they set
+  // every row's theta from those tiles' smallest half-block minima without
+  // recording anything, so recording starts against a theta near the row's
+  // final one (a streaming top-kTopK records about kTopK ln(columns seen /
+  // columns of the dry run) times while theta tightens). The remaining steps
+  // visit every tile, the dry run's again.
+  const int dryTiles = iters > 0 ? min(a.dryTiles, iters) : 0;
+  const int totalSteps = dryTiles + iters;
+  auto tileOfStep = [&](int j) { return tileOf(j < dryTiles ? j : j - dryTiles); };
+  if (totalSteps > 0) {
+    request(tileOfStep(0), 0);
+    if (totalSteps > 1)
+      request(tileOfStep(1), 1);
   }
-  // Iteration -1 is a dry run over the first tile: it sets every row's theta
-  // from that tile's minimum without recording anything, so the first tile's
-  // blocks are recorded against a real theta instead of an unbounded one.
-  for (int i = iters > 0 ? -1 : 0; i < iters; ++i) {
-    const bool dry = i < 0;
-    const int ii = dry ? 0 : i;
-    const int t = tileOf(ii), buf = ii % STAGES;
+  for (int j = 0; j < totalSteps; ++j) {
+    const bool dry = j < dryTiles;
+    const bool lastDry = j + 1 == dryTiles;
+    const int ii = dry ? j : j - dryTiles;
+    const int t = tileOfStep(j), buf = j % STAGES;
     // Tile t's loads are this wave's oldest outstanding ones; only the next
     // tile's (LOADS + 1) may remain (memory operations complete in order).
-    if (ii + 1 < iters)
+    if (j + 1 < totalSteps)
       waitVmcnt<LOADS + 1>();
     else
       waitVmcnt<0>();
     // Every wave's slices of tile t are in LDS, and every wave is done with
     // the previous tile, whose buffer the tile after next now reuses.
     __syncthreads();
-    if (!dry && i + 2 < iters)
-      request(tileOf(i + 2), (i + 2) % STAGES);
+    if (j + 2 < totalSteps)
+      request(tileOfStep(j + 2), (j + 2) % STAGES);
     const int c0 = t * kTileCols, c1 = c0 + kTileCols;
     // Wave-uniform decisions about the same-component mask.
     bool inter = false, contained = true;
@@ -587,11 +596,10 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
             record |= fminf(mA[rb], mB[rb]) <= rl[rb].th;
           }
           // One wave-uniform branch for the four row blocks: in the common
-          // case no lane records anything. In the dry run every valid row's
-          // theta is FLT_MAX, so every finite minimum is kept and nothing is
-          // recorded. (The minima are updated in this one branch only: a
-          // second update path would make the compiler copy them between
-          // registers every iteration.)
+          // case no lane records anything. The dry run only keeps minima
+          // (theta is FLT_MAX until its first merge). (The minima are updated
+          // in this one branch only: a second update path would make the
+          // compiler copy them between registers every iteration.)
           if (__ballot(record) != 0) {
 #pragma unroll
             for (int rb = 0; rb < 4; ++rb)
@@ -604,15 +612,15 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
         runTile(std::true_type{});
       else
         runTile(std::false_type{});
-      // Tile end (every second tile, the dry run and the last tile): both
-      // halves of each row agree on the row's kTopK smallest half-block minima
-      // and theta. The merged list stays with the lane of half 0 and half 1
-      // starts empty, so no minimum is counted twice; after the dry run both
-      // start empty (its tile is visited again). Between merges theta stays
-      // that of the last merge, only looser: more blocks are recorded, none
-      // is missed. (Every fourth tile records so many more that lists
-      // overflow.)
-      const bool mergeNow = dry || (ii & 1) || ii + 1 == iters;
+      // Tile end (every second tile, the end of the dry run and the last
+      // tile): both halves of each row agree on the row's kTopK smallest
+      // half-block minima and theta. The merged list stays with the lane of
+      // half 0 and half 1 starts empty, so no minimum is counted twice; after
+      // the dry run both start empty (its tiles are visited again). Between
+      // merges theta stays that of the last merge, only looser: more blocks
+      // are recorded, none is missed. (Every fourth tile records so many more
+      // that lists overflow.)
+      const bool mergeNow = lastDry || (ii & 1) || (!dry && ii + 1 == iters);
 #pragma unroll
       for (int rb = 0; rb < 4 && mergeNow; ++rb) {
         float c[kTopK];
@@ -630,7 +638,7 @@ __global__ void __launch_bounds__(256, 2) mfmaCandidateKernel(CandArgs a) {
         cx(c[1], c[3]);
         cx(c[0], c[1]);
         cx(c[2], c[3]);
-        const bool keep = !dry && hi == 0;
+        const bool keep = !lastDry && hi == 0;
 #pragma unroll
         for (int q = 0; q < kTopK; ++q)
           rl[rb].top[q] = keep ? c[q] : INFINITY;
@@ -1192,6 +1200,10 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   // The first round keeps each point's kTopK nearest, which settle most rows
   // of the second round; later rounds keep the nearest only.
   ca.rank = firstRound ? kTopK : 1;
+  // The first round keeps the kTopK nearest of every point: a longer dry run
+  // (kDryTilesFirstRound tiles, 0.4% of its columns) starts its records much
+  // closer to their final theta. Later rounds search few rows.
+  ca.dryTiles = firstRound ? kDryTilesFirstRound : 1;
   const int blocks = (numRows + kBlockRows - 1) / kBlockRows;
   // Column splits (each keeps at least 2,048 columns): the fewest splits
   // whose workgroups fill the last wave of resident slots (two per CU) at
