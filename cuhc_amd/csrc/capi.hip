@@ -99,6 +99,7 @@ int linkageImpl(const float *X, bool onDevice, int n, int d, int metricId, float
     opt.prune = prune && pts.ordered();
     opt.orig = pts.ordered() ? pts.orig.get() : nullptr;
     opt.stats = &stats;
+    opt.mixedFirstPass = useMixed(kEuclidean, d) && mixedPassSupports(d);
     if (s.ok())
       s(rnnWard(pts.data(), n, d, opt, tree));
     std::lock_guard<std::mutex> lock(gStatsMutex);
@@ -239,6 +240,7 @@ int cuhc_ward_rnn_with_options(const float *X, int n, int d, float minProgress, 
   opt.minProgress = minProgress;
   opt.denseBelow = denseBelow;
   opt.allowDense = allowDense != 0;
+  opt.mixedFirstPass = useMixed(kEuclidean, d) && mixedPassSupports(d);
   if (s.ok() && n > 1)
     s(rnnWard(pts.data(), n, d, opt, tree));
   return s.code;

@@ -37,6 +37,10 @@ struct RnnOptions {
   const int *orig = nullptr;
   bool prune = false;
   PassStats *stats = nullptr;
+  // The first pass (every cluster a single point, d <= 64) through the FP16
+  // MFMA candidates and the FP32 recheck: the Ward weight of two points is
+  // exactly their FP32 squared distance, so the keys equal the FP32 pass's.
+  bool mixedFirstPass = false;
 };
 
 // Ward through rounds of reciprocal nearest neighbours (FP64 centroids and

@@ -574,7 +574,18 @@ int rnnWard(const float *X, int n, int d, const RnnOptions &opt, double *tree) {
     WeightSpec w;
     w.kind = kWeightWard;
     w.size = cur.size;
-    if (pruneThisPass) {
+    if (m == n && opt.mixedFirstPass) {
+      // Every cluster is one point (size 1, component = slot): the Ward
+      // weight 2 * 1 * 1 / (1 + 1) * sq is sq exactly, and the mixed pass
+      // returns every point's exact (FP32 squared distance, slot) nearest.
+      int created = 0;
+      MixedPass *mp = mixedPassCreate(cur.c32, m, d, &created);
+      s(created);
+      if (mp) {
+        s(mixedPassRun(mp, comp, m, true, true, nearest));
+        mixedPassDestroy(mp);
+      }
+    } else if (pruneThisPass) {
       double share = 1.0;
       s(nearestOtherWalk(cur.c32, m, d, MetricSpec{kSqEuclidean, 2.0f}, w, comp, nullptr,
                          nullptr, nearest, &share, opt.stats));
