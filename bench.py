@@ -86,7 +86,11 @@ def calibrate_copy(mib):
 MFMA_PROBE_SHAPES = {0: "32x32x16 f16, accumulate in place",
                      1: "16x16x32 f16, accumulate in place",
                      2: "candidate loop: seeded 32x32x16, LDS operands, min epilogue",
-                     3: "candidate loop on 16x16x32: seeded, LDS operands, min epilogue"}
+                     3: "candidate loop on 16x16x32: seeded, LDS operands, min epilogue",
+                     4: "candidate loop on 32x32x16, 2 row blocks, 4 waves per SIMD",
+                     5: "candidate loop on 16x16x32, 4 row blocks, 4 waves per SIMD",
+                     6: "symmetric candidate loop on 32x32x16 (row and column tests), "
+                        "flops counted twice"}
 
 
 def mfma_ceiling(workgroups=2048, iters=4096):

@@ -165,8 +165,10 @@ int cuhc_copy_device(void *dst, const void *src, unsigned long long bytes);
  * running `iters` steps of one instruction mix (shape 0: 32x32x16 FP16 MFMAs
  * accumulating in place; 1: 16x16x32 FP16 MFMAs accumulating in place; 2: the
  * candidate pass's loop on synthetic LDS operands, seeded accumulators and the
- * per-lane minimum epilogue; 3: that loop on 16x16x32 MFMAs) and returns ms
- * and flops per launch. No memory
+ * per-lane minimum epilogue; 3: that loop on 16x16x32 MFMAs; 4 and 5: loops
+ * 2 and 3 with fewer row blocks per wave at four waves per SIMD; 6: loop 2
+ * with per-column tests as well, its flops counted twice) and returns ms
+ * and flops per launch (-2 for a shape outside 0-6). No memory
  * traffic; the benchmark uses it to state an attainable MFMA ceiling. No
  * counterpart in the reference. */
 int cuhc_mfma_probe(int shape, int workgroups, int iters, float *msPerLaunch,

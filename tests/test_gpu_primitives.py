@@ -98,14 +98,17 @@ def test_radix_sort_zero_bits_copies():
     check_sort(keys, 0, wide=False)
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5, 6])
 def test_mfma_probe_runs(shape):
     """The MFMA issue-rate probe runs each shape and counts its flops exactly
-    (16 MFMAs per wave per step: 32,768 flops each, 16,384 for the 16x16x32
-    loop); a bad shape is refused."""
+    (16 MFMAs per wave per step, 8 for shapes 4 and 5; 32,768 flops each,
+    16,384 for the 16x16x32 loops; the symmetric shape 6 counts each value
+    twice); a bad shape is refused."""
     ms, flops = ctypes.c_float(), ctypes.c_double()
     print(f"[primitives] mfma probe shape={shape}", flush=True)
     _lib.check(_lib.lib.cuhc_mfma_probe(shape, 8, 16, ctypes.byref(ms), ctypes.byref(flops)))
     assert ms.value > 0.0
-    assert flops.value == 8 * 4 * 16 * 16 * (16384.0 if shape == 3 else 32768.0)
-    assert _lib.lib.cuhc_mfma_probe(4, 8, 16, ctypes.byref(ms), ctypes.byref(flops)) != 0
+    per_step = 8 if shape in (4, 5) else 16
+    per_mfma = 16384.0 if shape in (3, 5) else 32768.0
+    assert flops.value == 8 * 4 * 16 * per_step * per_mfma * (2 if shape == 6 else 1)
+    assert _lib.lib.cuhc_mfma_probe(7, 8, 16, ctypes.byref(ms), ctypes.byref(flops)) != 0
