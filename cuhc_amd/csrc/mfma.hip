@@ -42,7 +42,7 @@
 // row's component (the entries were the nearest outside a subset of the new
 // component, so the first such entry is exact); only rows whose whole list
 // joined their component are searched again. Rows whose list overflows are
-// rechecked in full. This
+// rechecked in full, each by as many waves as 4,096-column chunks. This
 // translation unit is compiled with -fno-honor-nans (inputs are validated
 // finite), which lets fminf map to v_min/v_min3 without canonicalisation.
 #include "mst.hpp"
@@ -71,6 +71,8 @@ constexpr int kBlockRows = 4 * kWaveRows;
 constexpr int kCandCap = 64; // list entries per (row, half)
 constexpr int kTopK = 4; // nearest points kept per row for the next round
 constexpr int kDryTilesFirstRound = 16;
+constexpr int kDryTilesLaterRounds = 4; // per column split
+constexpr int kMaxSplits = 128;         // column splits of a later round
 
 std::atomic<int> gLayoutChecked{0}; // 0 unknown, 1 ok, -1 mismatch
 unsigned long long gStats[4];
@@ -707,7 +709,12 @@ struct RecheckArgs {
   // [2] pairs, [3] overflow rows (striped so that a million waves do not
   // serialise on one address).
   unsigned long long *stats;
+  int *overflowRows;  // rows whose list overflowed, appended at overflowCount
+  int *overflowCount; // zero before the recheck
 };
+
+// Columns one wave scans for a row whose list overflowed.
+constexpr int kOverflowWaveCols = 4096;
 
 constexpr int kStatStripes = 64;
 constexpr int kStripeWords = 16;
@@ -751,6 +758,48 @@ DEV unsigned long long recheckColumn(const RecheckArgs &a, const float *xi, int 
   return w != INFINITY ? packKey(w, (unsigned)j) : kEmptyKey;
 }
 
+// Sorted insertion of key into this lane's kTopK smallest keys (ascending).
+DEV void insertKey(unsigned long long (&best)[kTopK], unsigned long long key) {
+#pragma unroll
+  for (int q = kTopK - 1; q > 0; --q)
+    best[q] = key < best[q] ? (key > best[q - 1] ? key : best[q - 1]) : best[q];
+  best[0] = minKey(best[0], key);
+}
+
+// The wave's kTopK smallest keys (every lane gets them): repeatedly the
+// minimum of the lanes' heads, popped from the lane that holds it (keys are
+// distinct: distinct indices).
+DEV void waveTopK(unsigned long long (&best)[kTopK], unsigned long long (&top)[kTopK]) {
+#pragma unroll
+  for (int q = 0; q < kTopK; ++q) {
+    unsigned long long m = best[0];
+#pragma unroll
+    for (int s2 = 32; s2 > 0; s2 >>= 1)
+      m = minKey(m, shflXorKey(m, s2));
+    top[q] = m;
+    if (m != kEmptyKey && best[0] == m) {
+#pragma unroll
+      for (int r2 = 0; r2 + 1 < kTopK; ++r2)
+        best[r2] = best[r2 + 1];
+      best[kTopK - 1] = kEmptyKey;
+    }
+  }
+}
+
+// A row's result: its first rank keys (every column that could precede them
+// was checked) and its nearest.
+DEV void writeRowKeys(const RecheckArgs &a, int pos, const unsigned long long (&top)[kTopK],
+                      int lane) {
+  if (lane < kTopK)
+    a.topk[(size_t)a.perm[pos] * kTopK + lane] = lane >= a.rank ? kEmptyKey
+                                                 : lane == 0   ? top[0]
+                                                 : lane == 1   ? top[1]
+                                                 : lane == 2   ? top[2]
+                                                               : top[3];
+  if (lane == 0)
+    a.nearest[a.perm[pos]] = top[0];
+}
+
 __global__ void __launch_bounds__(256) recheckKernel(RecheckArgs a) {
   static_assert(kCandCap % 32 == 0, "lists are read 32 entries per half at a time");
   __shared__ float rowX[4][64];
@@ -762,6 +811,13 @@ __global__ void __launch_bounds__(256) recheckKernel(RecheckArgs a) {
   // of each half (slots past a count hold stale data and are masked below).
   const int pos = a.rows[3 * r], cs = a.rows[3 * r + 1], ce = a.rows[3 * r + 2];
   const int c0 = a.candCount[(size_t)r * 2], c1 = a.candCount[(size_t)r * 2 + 1];
+  if (c0 > kCandCap || c1 > kCandCap) {
+    // The list lost entries: the row is scanned in full by the overflow
+    // kernels (wave-uniform exit).
+    if (lane == 0)
+      a.overflowRows[atomicAdd(a.overflowCount, 1)] = r;
+    return;
+  }
   const float th = thetaFromBits(a.theta[r]);
   const int half = lane >> 5, e32 = lane & 31;
   const size_t listBase = ((size_t)r * 2 + half) * kCandCap;
@@ -777,18 +833,8 @@ __global__ void __launch_bounds__(256) recheckKernel(RecheckArgs a) {
 #pragma unroll
   for (int q = 0; q < kTopK; ++q)
     best[q] = kEmptyKey;
-  auto insert = [&](unsigned long long key) {
-#pragma unroll
-    for (int q = kTopK - 1; q > 0; --q)
-      best[q] = key < best[q] ? (key > best[q - 1] ? key : best[q - 1]) : best[q];
-    best[0] = minKey(best[0], key);
-  };
   unsigned long long blocks = 0;
-  const bool overflow = c0 > kCandCap || c1 > kCandCap;
-  if (overflow) {
-    for (int col = lane; col < a.n; col += 64)
-      insert(recheckColumn(a, xi, col, cs, ce));
-  } else {
+  {
     const int sub = lane >> 4, idx = lane & 15;
     const int count = half ? c1 : c0, longest = max(c0, c1);
     // 32 entries of each half at a time.
@@ -813,46 +859,72 @@ __global__ void __launch_bounds__(256) recheckKernel(RecheckArgs a) {
           // Half-block b (32-column block b >> 1, lane half b & 1) covers
           // columns (b >> 1) * 32 + g * 8 + (b & 1) * 4 + q, g, q in 0..3.
           const int col = (blk >> 1) * 32 + (idx >> 2) * 8 + (blk & 1) * 4 + (idx & 3);
-          insert(recheckColumn(a, xi, col, cs, ce));
+          insertKey(best, recheckColumn(a, xi, col, cs, ce));
         }
       }
     }
   }
-  // The wave's kTopK smallest: repeatedly the minimum of the lanes' heads,
-  // popped from the lane that holds it (keys are distinct: distinct indices).
   unsigned long long top[kTopK];
-#pragma unroll
-  for (int q = 0; q < kTopK; ++q) {
-    unsigned long long m = best[0];
-#pragma unroll
-    for (int s2 = 32; s2 > 0; s2 >>= 1)
-      m = minKey(m, shflXorKey(m, s2));
-    top[q] = m;
-    if (m != kEmptyKey && best[0] == m) {
-#pragma unroll
-      for (int r2 = 0; r2 + 1 < kTopK; ++r2)
-        best[r2] = best[r2 + 1];
-      best[kTopK - 1] = kEmptyKey;
-    }
-  }
-  // Only the first rank keys are certified (every column that could precede
-  // them was a candidate).
-  if (lane < kTopK)
-    a.topk[(size_t)a.perm[pos] * kTopK + lane] = lane >= a.rank ? kEmptyKey
-                                                 : lane == 0   ? top[0]
-                                                 : lane == 1   ? top[1]
-                                                 : lane == 2   ? top[2]
-                                                               : top[3];
-  if (lane == 0) {
-    a.nearest[a.perm[pos]] = top[0];
+  waveTopK(best, top);
+  writeRowKeys(a, pos, top, lane);
+  if (lane == 0 && blocks) {
     unsigned long long *st = a.stats + (size_t)(r % kStatStripes) * kStripeWords;
-    if (overflow) {
-      atomicAdd(&st[3], 1ull);
-      atomicAdd(&st[2], (unsigned long long)a.n);
-    } else if (blocks) {
-      atomicAdd(&st[1], blocks);
-      atomicAdd(&st[2], blocks * 16);
-    }
+    atomicAdd(&st[1], blocks);
+    atomicAdd(&st[2], blocks * 16);
+  }
+}
+
+// Rows whose list overflowed: gridDim.y rows (overflowRows[first ..]), each
+// scanned by gridDim.x workgroups of four waves, wave w taking columns
+// [w * kOverflowWaveCols, (w + 1) * kOverflowWaveCols) and leaving its kTopK
+// smallest keys in partial[(row * waves + w) * kTopK ..].
+__global__ void __launch_bounds__(256) overflowScanKernel(RecheckArgs a, int first,
+                                                         unsigned long long *partial) {
+  __shared__ float rowX[64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int o = blockIdx.y, r = a.overflowRows[first + o];
+  const int pos = a.rows[3 * r], cs = a.rows[3 * r + 1], ce = a.rows[3 * r + 2];
+  for (int k = threadIdx.x; k < a.d; k += 256)
+    rowX[k] = a.xp[(size_t)pos * a.d + k];
+  __syncthreads();
+  const int waves = gridDim.x * 4, w = blockIdx.x * 4 + wave;
+  const long long col0 = (long long)w * kOverflowWaveCols;
+  const int end = (int)min((long long)a.n, col0 + kOverflowWaveCols);
+  unsigned long long best[kTopK];
+#pragma unroll
+  for (int q = 0; q < kTopK; ++q)
+    best[q] = kEmptyKey;
+  for (long long col = col0 + lane; col < end; col += 64)
+    insertKey(best, recheckColumn(a, rowX, (int)col, cs, ce));
+  unsigned long long top[kTopK];
+  waveTopK(best, top);
+  if (lane < kTopK)
+    partial[((size_t)o * waves + w) * kTopK + lane] = lane == 0   ? top[0]
+                                                      : lane == 1 ? top[1]
+                                                      : lane == 2 ? top[2]
+                                                                  : top[3];
+}
+
+// One wave per overflowed row: the kTopK smallest of its waves' keys.
+__global__ void __launch_bounds__(64) overflowMergeKernel(RecheckArgs a, int first,
+                                                         const unsigned long long *partial,
+                                                         int waves) {
+  const int lane = threadIdx.x, o = blockIdx.x, r = a.overflowRows[first + o];
+  const int pos = a.rows[3 * r];
+  unsigned long long best[kTopK];
+#pragma unroll
+  for (int q = 0; q < kTopK; ++q)
+    best[q] = kEmptyKey;
+  const unsigned long long *keys = partial + (size_t)o * waves * kTopK;
+  for (int i = lane; i < waves * kTopK; i += 64)
+    insertKey(best, keys[i]);
+  unsigned long long top[kTopK];
+  waveTopK(best, top);
+  writeRowKeys(a, pos, top, lane);
+  if (lane == 0) {
+    unsigned long long *st = a.stats + (size_t)(r % kStatStripes) * kStripeWords;
+    atomicAdd(&st[3], 1ull);
+    atomicAdd(&st[2], (unsigned long long)a.n);
   }
 }
 
@@ -1012,6 +1084,7 @@ struct MixedPass {
   size_t sortBytes = 0, scanBytes = 0;
   DevArray<uint2> cand;
   DevArray<unsigned long long> stats, recheckStats, topk, largest;
+  DevArray<int> overflowRows, overflowCount;
   float Rmax = 0.0f, rhoMax = 0.0f;
 };
 
@@ -1050,6 +1123,8 @@ MixedPass *mixedPassCreate(const float *X, int n, int d, int *status) {
   s(mp->largest.alloc(1));
   s(mp->stats.alloc(5));
   s(mp->recheckStats.alloc((size_t)kStatStripes * kStripeWords));
+  s(mp->overflowRows.alloc((size_t)2 * n)); // one entry per row slot at most
+  s(mp->overflowCount.alloc(1));
   s(mp->positions.alloc(n));
   s(mp->sortedComp.alloc(n));
   s(mp->compStart.alloc(n));
@@ -1202,16 +1277,19 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   ca.rank = firstRound ? kTopK : 1;
   // The first round keeps the kTopK nearest of every point: a longer dry run
   // (kDryTilesFirstRound tiles, 0.4% of its columns) starts its records much
-  // closer to their final theta. Later rounds search few rows.
-  ca.dryTiles = firstRound ? kDryTilesFirstRound : 1;
+  // closer to their final theta. Later rounds search few rows over many
+  // column splits; each split's dry run of kDryTilesLaterRounds tiles keeps
+  // its records, before the splits share theta, within the lists.
+  ca.dryTiles = firstRound ? kDryTilesFirstRound : kDryTilesLaterRounds;
   const int blocks = (numRows + kBlockRows - 1) / kBlockRows;
-  // Column splits (each keeps at least 2,048 columns): the fewest splits
-  // whose workgroups fill the last wave of resident slots (two per CU) at
-  // least 90%, else the best fill. Late rounds with few rows would otherwise
-  // leave CUs idle, and a grid just above a multiple of the resident slots
-  // would run its last wave nearly empty.
+  // Column splits (at most kMaxSplits, each keeping at least 2,048 columns):
+  // the fewest splits whose workgroups fill the last wave of resident slots
+  // (two per CU) at least 90%, else the best fill. Late rounds with few rows
+  // would otherwise leave CUs idle (rounds 2-5 at 10^6 points: 6.6 ms with
+  // at most 16 splits, about 4.3 ms with 128), and a grid just above a
+  // multiple of the resident slots would run its last wave nearly empty.
   const long long slots = 2LL * numCUs();
-  const int maxSplits = std::max(1, std::min(16, mp->nPad / 2048));
+  const int maxSplits = std::max(1, std::min(kMaxSplits, mp->nPad / 2048));
   int splits = 1;
   double bestFill = 0.0;
   for (int sp = 1; sp <= maxSplits; ++sp) {
@@ -1260,10 +1338,33 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   ra.topk = mp->topk;
   ra.rank = ca.rank;
   ra.stats = mp->recheckStats;
+  ra.overflowRows = mp->overflowRows;
+  ra.overflowCount = mp->overflowCount;
+  s(fill(mp->overflowCount, 0, sizeof(int)));
+  int overflowed = 0;
   {
     PhaseTimer timer(kPhaseRecheck);
     hipLaunchKernelGGL(recheckKernel, dim3((numRows + 3) / 4), dim3(256), 0, stream(), ra);
     s((int)hipGetLastError());
+  }
+  s(download(&overflowed, mp->overflowCount, sizeof(int)));
+  if (s.ok() && overflowed > 0) {
+    // Rows whose lists overflowed, at most 1,024 per batch and at most 32 MiB
+    // of partial keys: every column, kOverflowWaveCols per wave.
+    PhaseTimer timer(kPhaseRecheck);
+    const int groups = (int)((((long long)n + kOverflowWaveCols - 1) / kOverflowWaveCols + 3) / 4);
+    const int waves = groups * 4;
+    const int batch = std::max(1, std::min({overflowed, 1024, (32 << 20) / (waves * kTopK * 8)}));
+    DevArray<unsigned long long> partial;
+    s(partial.alloc((size_t)batch * waves * kTopK));
+    for (int first = 0; s.ok() && first < overflowed; first += batch) {
+      const int rowsNow = std::min(batch, overflowed - first);
+      hipLaunchKernelGGL(overflowScanKernel, dim3(groups, rowsNow), dim3(256), 0, stream(), ra,
+                         first, partial.get());
+      hipLaunchKernelGGL(overflowMergeKernel, dim3(rowsNow), dim3(64), 0, stream(), ra, first,
+                         partial.get(), waves);
+      s((int)hipGetLastError());
+    }
   }
   return s.code;
 }

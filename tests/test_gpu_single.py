@@ -51,6 +51,21 @@ def test_mixed_on_clustered_and_offset_data():
     np.testing.assert_array_equal(run(Y, precision="mixed"), run(Y, precision="fp32"))
 
 
+def test_mixed_overflowing_lists():
+    # 150 copies of each of 200 points, shuffled: every row has 149 columns at
+    # weight 0 spread over about as many half-blocks, more than a candidate
+    # list holds, so its rows are rechecked by the chunked full scan.
+    rng = np.random.default_rng(8)
+    base = rng.normal(size=(200, 16)).astype(np.float32)
+    X = np.repeat(base, 150, axis=0)[rng.permutation(30_000)]
+    print("[single] overflowing lists n=30000 d=16", flush=True)
+    _lib.mixed_stats()
+    Z = run(X, precision="mixed")
+    stats = _lib.mixed_stats()
+    assert stats["overflow_rows"] > 0, stats
+    np.testing.assert_array_equal(Z, run(X, precision="fp32"))
+
+
 @pytest.mark.parametrize("pruning", ["never", "always"])
 def test_majority_component_rounds(pruning):
     # One dense cloud holding most points and a few small far groups: once the
