@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace cuhc {
 
@@ -73,16 +74,22 @@ bool mergesToTreeOrder(std::vector<Edge> &merges, int n) {
 
 namespace {
 
-// Union-find for the ordered merges: a parent link per point (-1 at a root)
-// and, per root, its cluster's size and scipy id in one 8-byte record, both
-// kept between calls: 4 bytes per point touched at random plus one record per
+// Union-find for the ordered merges: a link per point and, per root of a
+// merged cluster, its size and scipy id in one 8-byte record, both kept
+// between calls: 4 bytes per point touched at random plus one record per
 // root, instead of a 24-byte node, halves the assembly time at 10^6 points.
+// A link is the parent (>= 0), kAlone for a point not merged yet (its own
+// root: size 1, id itself, no record read) or kMerged for the root of a
+// merged cluster, so only the links are initialised (one memset) and most
+// merges, which take in a point still alone, read one record instead of two:
+// 10^6 points after 80 ms of GPU work, 11-12 ms against 8.5-9 ms.
 // Finds halve the path as they climb; the links of the endpoints 16 merges
 // ahead are prefetched. With ORDER_TIES, a run of merges with equal (weight,
 // u) is sorted by v when the loop reaches it (the run is read anyway).
 struct RootInfo {
   int size, id;
 };
+constexpr int kAlone = -1, kMerged = -2;
 
 template <bool ORDER_TIES, class E> void assembleTree(E *merges, size_t m, int n, double *tree) {
   static thread_local std::vector<int> link;
@@ -91,12 +98,10 @@ template <bool ORDER_TIES, class E> void assembleTree(E *merges, size_t m, int n
     link.resize(n);
     root.resize(n);
   }
-  for (int i = 0; i < n; ++i) {
-    link[i] = -1;
-    root[i] = {1, i};
-  }
   int *up = link.data();
   RootInfo *info = root.data();
+  static_assert(kAlone == -1, "all-ones bytes");
+  std::memset(up, 0xff, (size_t)n * sizeof(int));
   auto find = [up](int x) {
     while (up[x] >= 0) {
       const int p = up[x];
@@ -124,7 +129,8 @@ template <bool ORDER_TIES, class E> void assembleTree(E *merges, size_t m, int n
       __builtin_prefetch(&up[merges[s + kAhead].v], 1);
     }
     int ra = find(merges[s].u), rb = find(merges[s].v);
-    const RootInfo ia = info[ra], ib = info[rb];
+    const RootInfo ia = up[ra] == kAlone ? RootInfo{1, ra} : info[ra];
+    const RootInfo ib = up[rb] == kAlone ? RootInfo{1, rb} : info[rb];
     const int total = ia.size + ib.size;
     double *row = tree + s * 4;
     row[0] = std::min(ia.id, ib.id);
@@ -134,6 +140,7 @@ template <bool ORDER_TIES, class E> void assembleTree(E *merges, size_t m, int n
     if (ia.size < ib.size)
       std::swap(ra, rb);
     up[rb] = ra;
+    up[ra] = kMerged;
     info[ra] = {total, n + (int)s};
   }
 }
