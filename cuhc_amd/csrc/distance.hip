@@ -7,6 +7,7 @@
 // estimatePrunedShare (src/spatial.cuh:136-343, 424-469). The tile fold is
 // tile.hpp; argmins carry (weight, index) as one u64 key.
 #include "passes.hpp"
+#include "primitives.hpp"
 #include "tile.hpp"
 
 #include <algorithm>
@@ -343,9 +344,13 @@ struct Boxes {
   int numTiles = 0;
 };
 
+// Boxes of the tiles of X (n points of d features, tileDim per tile). The
+// weight values and components of point p are those of position pos[p]
+// (pos null: p), for rows gathered out of the full order.
 template <class Weight>
 __global__ void buildBoxesKernel(const float *__restrict__ X, int n, int d, int tileDim,
-                                 Weight weight, const int *__restrict__ comp, Boxes boxes) {
+                                 Weight weight, const int *__restrict__ comp,
+                                 const int *__restrict__ pos, Boxes boxes) {
   const int tile = blockIdx.x;
   const int first = tile * tileDim, last = min(n, first + tileDim);
   for (int f = threadIdx.x; f < d; f += blockDim.x) {
@@ -363,8 +368,9 @@ __global__ void buildBoxesKernel(const float *__restrict__ X, int n, int d, int 
     float mv = INFINITY;
     int cmin = 0x7fffffff, cmax = -1;
     for (int p = first + (int)threadIdx.x; p < last; p += 64) {
-      mv = fminf(mv, (float)weight.value(p));
-      const int c = comp ? comp[p] : 0;
+      const int q = pos ? pos[p] : p;
+      mv = fminf(mv, (float)weight.value(q));
+      const int c = comp ? comp[q] : 0;
       cmin = min(cmin, c);
       cmax = max(cmax, c);
     }
@@ -458,7 +464,18 @@ template <class Weight> struct ColumnStage {
 // LDS, and for metrics with an accumulator bound a pair is finished (sqrt,
 // weight, key) only when its raw accumulator is within the row's need and
 // it crosses components; the pairs filtered out weigh more than the need.
-template <class T, class Metric, class Weight>
+// The rows of a compact walk: n rows gathered into X (d features each) from
+// positions pos[0 .. n) of the full order (ascending), in tiles with their
+// own boxes; center[t] is the full-order tile around which row tile t walks.
+struct RowSet {
+  const float *X = nullptr;
+  int n = 0;
+  const int *pos = nullptr;
+  Boxes boxes;
+  const int *center = nullptr;
+};
+
+template <class T, class Metric, class Weight, bool COMPACT = false>
 __global__ void __launch_bounds__(256)
     nearestOtherWalkKernel(const float *__restrict__ X, int n, int d, Metric metric,
                            Weight weight, const int *__restrict__ comp,
@@ -467,8 +484,15 @@ __global__ void __launch_bounds__(256)
                            unsigned long long *__restrict__ nearest,
                            unsigned long long *tilesComputed, int rowTileStride,
                            int computeSteps, const int *__restrict__ skipComp,
-                           const unsigned char *__restrict__ closedRows) {
+                           const unsigned char *__restrict__ closedRows, RowSet rows) {
   constexpr int TR = T::TR;
+  // Rows: the full order's tiles, or (COMPACT) the gathered rows, whose
+  // position in the full order indexes comp, the weight values, closedRows
+  // and nearest.
+  const float *__restrict__ rowX = COMPACT ? rows.X : X;
+  const int nRows = COMPACT ? rows.n : n;
+  const Boxes &rowBoxes = COMPACT ? rows.boxes : boxes;
+  auto posOf = [&](int row) { return COMPACT ? rows.pos[row] : row; };
   constexpr int kBatch = 64;
   __shared__ __align__(16) TileSmem<T> sm;
   __shared__ unsigned long long rowBest[T::DIM];
@@ -484,15 +508,19 @@ __global__ void __launch_bounds__(256)
 
   const int rowTile = blockIdx.x * rowTileStride, numTiles = boxes.numTiles;
   const int row0 = rowTile * T::DIM;
+  const int center = COMPACT ? rows.center[rowTile] : rowTile;
   // Rows of the skipped component are closed; a tile of only such rows has
   // nothing to do (uniform: every thread reads the same values).
   const int skip = skipComp ? *skipComp : -1;
-  if (skip >= 0 && boxes.minComp[rowTile] == skip && boxes.maxComp[rowTile] == skip)
+  if (skip >= 0 && rowBoxes.minComp[rowTile] == skip && rowBoxes.maxComp[rowTile] == skip)
     return;
   // A row is open unless it lies past the end, in the skipped component, or
   // is closed (its exact answer is already in nearest[]).
   auto isOpen = [&](int row) {
-    return row < n && comp[row] != skip && !(closedRows && closedRows[row]);
+    if (row >= nRows)
+      return false;
+    const int p = posOf(row);
+    return comp[p] != skip && !(closedRows && closedRows[p]);
   };
   int anyOpen = 0;
   for (int i = threadIdx.x; i < T::DIM; i += T::THREADS) {
@@ -504,24 +532,24 @@ __global__ void __launch_bounds__(256)
   if (!__syncthreads_or(anyOpen))
     return; // uniform: every row of the tile is settled
   for (int k = threadIdx.x; k < d; k += T::THREADS) {
-    rowBox[k] = boxes.low[(size_t)rowTile * d + k];
-    rowBox[d + k] = boxes.high[(size_t)rowTile * d + k];
+    rowBox[k] = rowBoxes.low[(size_t)rowTile * d + k];
+    rowBox[d + k] = rowBoxes.high[(size_t)rowTile * d + k];
   }
   // Wave 0 owns rows lane and lane + 64 for the bound: their components and
   // the components' best known edges (refreshed once per batch).
   const int ra = row0 + (int)(threadIdx.x & 63), rb = ra + 64;
   const bool open0 = threadIdx.x < 64 && isOpen(ra);
   const bool open1 = threadIdx.x < 64 && isOpen(rb);
-  const int comp0 = open0 ? comp[ra] : 0, comp1 = open1 ? comp[rb] : 0;
+  const int comp0 = open0 ? comp[posOf(ra)] : 0, comp1 = open1 ? comp[posOf(rb)] : 0;
   float cap0 = INFINITY, cap1 = INFINITY;
   auto capOf = [&](unsigned int bits) { return bits == 0xffffffffu ? INFINITY : floatOf(bits); };
   if (threadIdx.x == 0) {
     bound = INFINITY;
     tileImproved = 0;
   }
-  const bool rowTilePure = boxes.minComp[rowTile] == boxes.maxComp[rowTile];
-  const int rowCompTile = boxes.minComp[rowTile];
-  const float rowMinValue = boxes.minValue[rowTile];
+  const bool rowTilePure = rowBoxes.minComp[rowTile] == rowBoxes.maxComp[rowTile];
+  const int rowCompTile = rowBoxes.minComp[rowTile];
+  const float rowMinValue = rowBoxes.minValue[rowTile];
   const float slack = 1.0f - boxSlack(d);
 
   int rowComp[TR];
@@ -529,15 +557,15 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int r = 0; r < TR; ++r) {
     const int row = row0 + T::rowInTile(r);
-    rowComp[r] = isOpen(row) ? comp[row] : -2;
-    rowVal[r] = weight.value(row < n ? row : 0);
+    rowComp[r] = isOpen(row) ? comp[posOf(row)] : -2;
+    rowVal[r] = weight.value(row < nRows ? posOf(row) : 0);
   }
   unsigned long long computed = 0;
   // The 8-feature tiles serve d <= 8 only (one fill): the row slice is
   // staged once for every tile.
   constexpr bool kRowsResident = T::FILL == 8;
   if (kRowsResident)
-    stageRows<T>(sm, X, n, d, row0);
+    stageRows<T>(sm, rowX, nRows, d, row0);
   __syncthreads();
 
   // The bound: the largest need over the open rows (inf while a row of a
@@ -587,7 +615,7 @@ __global__ void __launch_bounds__(256)
       }
       // -1: never computed (outside the grid, or both tiles wholly one component).
       const int step = base + (int)threadIdx.x;
-      const int colTile = step < steps ? walkTile(rowTile, step, numTiles) : -1;
+      const int colTile = step < steps ? walkTile(center, step, numTiles) : -1;
       float lb = -1.0f;
       if (colTile >= 0 &&
           !(rowTilePure && boxes.minComp[colTile] == boxes.maxComp[colTile] &&
@@ -603,7 +631,7 @@ __global__ void __launch_bounds__(256)
       const float lb = lowerBound[j];
       if (lb < 0.0f || lb > bound)
         continue; // uniform: lowerBound and bound are shared
-      const int colTile = walkTile(rowTile, base + j, numTiles);
+      const int colTile = walkTile(center, base + j, numTiles);
       ++computed;
       if (base + j >= computeSteps)
         continue; // estimate: counted against the bound reached so far, not computed
@@ -613,8 +641,8 @@ __global__ void __launch_bounds__(256)
       // computeTile's barriers; rewritten only after this tile's last barrier).
       const ColumnStage<Weight> stage{comp, orig, weight, col0, n, T::DIM,
                                       sColComp, sColVal, sColIdx};
-      computeTile<T, Metric, !kFilter, kRowsResident>(sm, X, n, X, n, d, metric, row0, col0, w,
-                                                      stage);
+      computeTile<T, Metric, !kFilter, kRowsResident>(sm, rowX, nRows, X, n, d, metric, row0,
+                                                      col0, w, stage);
       int colComp[TR];
       typename Weight::Value colVal[TR];
       unsigned int colIdx[TR];
@@ -683,8 +711,8 @@ __global__ void __launch_bounds__(256)
     __syncthreads(); // lowerBound is rewritten by the next batch
   }
   for (int i = threadIdx.x; i < T::DIM; i += T::THREADS)
-    if (row0 + i < n && rowBest[i] != kEmptyKey)
-      atomicMin(&nearest[row0 + i], rowBest[i]);
+    if (row0 + i < nRows && rowBest[i] != kEmptyKey)
+      atomicMin(&nearest[posOf(row0 + i)], rowBest[i]);
   if (tilesComputed && threadIdx.x == 0)
     atomicAdd(tilesComputed, computed);
 }
@@ -965,7 +993,7 @@ int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &we
     return s.code;
   s(fill(counter, 0, sizeof(unsigned long long)));
   hipLaunchKernelGGL((buildBoxesKernel<Weight>), dim3(tiles), dim3(64), 0, stream(), X, n, d,
-                     T::DIM, weight, comp, boxes.view);
+                     T::DIM, weight, comp, nullptr, boxes.view);
   s((int)hipGetLastError());
   const int sampled = (tiles + stride - 1) / stride;
   {
@@ -973,13 +1001,123 @@ int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &we
     hipLaunchKernelGGL((nearestOtherWalkKernel<T, Metric, Weight>), dim3(sampled), dim3(256),
                        2 * d * sizeof(float), stream(), X, n, d, metric, weight, comp, orig,
                        boxes.view, compBest, nearest, counter.get(), stride,
-                       stride > 1 ? kEstimateSteps : 0x7fffffff, skipComp, closedRows);
+                       stride > 1 ? kEstimateSteps : 0x7fffffff, skipComp, closedRows,
+                       RowSet{});
     s((int)hipGetLastError());
   }
   unsigned long long computed = 0;
   s(download(&computed, counter, sizeof(computed)));
   const unsigned long long total = (unsigned long long)sampled * tiles;
   *share = total ? (double)computed / (double)total : 1.0;
+  addUnits(kPhaseDistanceTiles, (long long)computed * T::DIM * T::DIM);
+  if (stats && s.ok()) {
+    stats->tilesComputed += computed;
+    stats->tilesTotal += total;
+  }
+  return s.code;
+}
+
+// Open rows of a walk: not closed and not in the skipped component.
+__global__ void openRowFlagsKernel(const int *__restrict__ comp, const int *__restrict__ skipComp,
+                                   const unsigned char *__restrict__ closedRows, int n,
+                                   int *flags) {
+  const int skip = skipComp ? *skipComp : -1;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p <= n; p += gridDim.x * blockDim.x)
+    flags[p] = p < n && comp[p] != skip && !closedRows[p];
+}
+
+__global__ void openRowListKernel(const int *__restrict__ flags, const int *__restrict__ index,
+                                  int n, int *rowPos) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
+    if (flags[p])
+      rowPos[index[p]] = p;
+}
+
+// The full-order tile of each compact row tile's middle row.
+__global__ void rowTileCentersKernel(const int *__restrict__ rowPos, int nOpen, int tileDim,
+                                     int rowTiles, int *center) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < rowTiles)
+    center[t] = rowPos[min(t * tileDim + tileDim / 2, nOpen - 1)] / tileDim;
+}
+
+// The walk over the open rows only, when closed rows are given: those rows
+// (in the full order, so spatially coherent) are gathered into compact row
+// tiles with their own boxes, each walking the column tiles around the full
+// order's tile of its middle row. With most rows closed, a tile of the full
+// order usually keeps a few open rows and walks as far as they need; the
+// compact tiles walk for open rows only. The share reported is of the full
+// grid's tile pairs.
+template <class T, class Metric, class Weight>
+int runWalkOpen(const float *X, int n, int d, const Metric &metric, const Weight &weight,
+                const int *comp, const int *orig, unsigned int *compBest,
+                unsigned long long *nearest, double *share, PassStats *stats,
+                const int *skipComp, const unsigned char *closedRows) {
+  const int tiles = (n + T::DIM - 1) / T::DIM;
+  Status s;
+  DevArray<int> flags, index, rowPos;
+  DevArray<unsigned char> temp;
+  s(flags.alloc((size_t)n + 1));
+  s(index.alloc((size_t)n + 1));
+  s(rowPos.alloc(n));
+  s(temp.alloc(scanTempBytes(n + 1)));
+  if (!s.ok())
+    return s.code;
+  const int blocks = std::max(1, std::min(4 * numCUs(), (n + 256) / 256));
+  hipLaunchKernelGGL(openRowFlagsKernel, dim3(blocks), dim3(256), 0, stream(), comp, skipComp,
+                     closedRows, n, flags.get());
+  s(exclusiveScan(flags, index, n + 1, temp));
+  int nOpen = 0;
+  s(download(&nOpen, index.get() + n, sizeof(int)));
+  if (!s.ok())
+    return s.code;
+  if (4LL * nOpen > 3LL * n) // most rows open: the full order's tiles
+    return runWalk<T>(X, n, d, metric, weight, comp, orig, compBest, nearest, share, stats, 1,
+                      skipComp, closedRows);
+  *share = 0.0;
+  if (nOpen == 0)
+    return 0;
+  hipLaunchKernelGGL(openRowListKernel, dim3(blocks), dim3(256), 0, stream(), flags.get(),
+                     index.get(), n, rowPos.get());
+  const int rowTiles = (nOpen + T::DIM - 1) / T::DIM;
+  DeviceBoxes colBoxes, rowBoxes;
+  DevArray<float> rowX;
+  DevArray<int> center;
+  DevArray<unsigned long long> counter;
+  s(colBoxes.alloc(tiles, d));
+  s(rowBoxes.alloc(rowTiles, d));
+  s(rowX.alloc((size_t)nOpen * d));
+  s(center.alloc(rowTiles));
+  s(counter.alloc(1));
+  if (!s.ok())
+    return s.code;
+  s(fill(counter, 0, sizeof(unsigned long long)));
+  s(gatherRows(X, rowPos, nOpen, d, rowX));
+  hipLaunchKernelGGL((buildBoxesKernel<Weight>), dim3(tiles), dim3(64), 0, stream(), X, n, d,
+                     T::DIM, weight, comp, nullptr, colBoxes.view);
+  hipLaunchKernelGGL((buildBoxesKernel<Weight>), dim3(rowTiles), dim3(64), 0, stream(),
+                     rowX.get(), nOpen, d, T::DIM, weight, comp, rowPos.get(), rowBoxes.view);
+  hipLaunchKernelGGL(rowTileCentersKernel, dim3((rowTiles + 255) / 256), dim3(256), 0, stream(),
+                     rowPos.get(), nOpen, T::DIM, rowTiles, center.get());
+  s((int)hipGetLastError());
+  RowSet rows;
+  rows.X = rowX;
+  rows.n = nOpen;
+  rows.pos = rowPos;
+  rows.boxes = rowBoxes.view;
+  rows.center = center;
+  {
+    PhaseTimer timer(kPhaseDistanceTiles);
+    hipLaunchKernelGGL((nearestOtherWalkKernel<T, Metric, Weight, true>), dim3(rowTiles),
+                       dim3(256), 2 * d * sizeof(float), stream(), X, n, d, metric, weight, comp,
+                       orig, colBoxes.view, compBest, nearest, counter.get(), 1, 0x7fffffff,
+                       skipComp, closedRows, rows);
+    s((int)hipGetLastError());
+  }
+  unsigned long long computed = 0;
+  s(download(&computed, counter, sizeof(computed)));
+  const unsigned long long total = (unsigned long long)tiles * tiles;
+  *share = (double)computed / (double)total;
   addUnits(kPhaseDistanceTiles, (long long)computed * T::DIM * T::DIM);
   if (stats && s.ok()) {
     stats->tilesComputed += computed;
@@ -1013,6 +1151,13 @@ int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, c
       return withWeightFor(metric, w, [&](auto weight) {
         return withFill(d, [&](auto fill) {
           using T = Tile<8, decltype(fill)::value>;
+          // Closed rows come with mutual-reachability weights (their answers
+          // settled from candidate lists): walk the open rows only.
+          if constexpr (decltype(weight)::kHasValue) {
+            if (closedRows && stride == 1)
+              return runWalkOpen<T>(X, n, d, metric, weight, comp, orig, compBest, nearest,
+                                    share, stats, skipComp, closedRows);
+          }
           return runWalk<T>(X, n, d, metric, weight, comp, orig, compBest, nearest, share,
                             stats, stride, skipComp, closedRows);
         });
