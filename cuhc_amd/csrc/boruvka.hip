@@ -103,14 +103,22 @@ int boruvkaMst(const float *X, int n, int d, MetricSpec m, WeightSpec w, const M
   // Rows settled from the learned candidate lists (mutual reachability,
   // alpha = 1) are closed to the walk.
   const bool useLists = opt.lists && opt.prune && w.kind == kWeightMReach && w.alpha == 1.0f;
+  // Without lists, a pruned mutual-reachability run has the first round's walk
+  // keep each row's 4 smallest keys over all other points, and settles the
+  // later rounds' rows from them.
+  const bool keepKeys = !useLists && opt.prune && w.kind == kWeightMReach;
   DevArray<unsigned char> dClosed;
-  DevArray<unsigned long long> dListCounters;
-  if (useLists) {
+  DevArray<unsigned long long> dListCounters, dKept;
+  if (useLists || keepKeys)
     s(dClosed.alloc(n));
+  if (keepKeys)
+    s(dKept.alloc((size_t)n * 4));
+  if (useLists) {
     s(dListCounters.alloc(64 * 16));
     if (s.ok())
       s(fill(dListCounters, 0, dListCounters.bytes()));
   }
+  bool keptValid = false; // dKept holds the first round's lists
   if (s.ok())
     s(resetComponents(dComp, n));
   MixedPass *mixed = nullptr;
@@ -138,8 +146,17 @@ int boruvkaMst(const float *X, int n, int d, MetricSpec m, WeightSpec w, const M
         if (useLists)
           s(settleFromLists(*opt.lists, n, dComp, opt.devOrig, w.core, dSkip, dNearest,
                             dCompBest, dClosed, dListCounters));
-        s(nearestOtherWalk(X, n, d, m, w, dComp, opt.devOrig, dCompBest, dNearest, &share,
-                           opt.stats, 1, dSkip, useLists ? dClosed.get() : nullptr));
+        if (keepKeys && !keptValid) {
+          s(nearestOtherWalk(X, n, d, m, w, dComp, opt.devOrig, nullptr, dNearest, &share,
+                             opt.stats, 1, dSkip, nullptr, dKept));
+          keptValid = true;
+        } else {
+          if (keepKeys)
+            s(settleFromKept(dKept, n, dComp, posOf, dSkip, dNearest, dCompBest, dClosed));
+          s(nearestOtherWalk(X, n, d, m, w, dComp, opt.devOrig, dCompBest, dNearest, &share,
+                             opt.stats, 1, dSkip,
+                             useLists || keepKeys ? dClosed.get() : nullptr));
+        }
         pruneThisPass = share <= kPruneGiveUpShare;
       } else {
         s(nearestOtherSymmetric(X, n, d, m, w, dComp, opt.devOrig, dNearest));

@@ -475,8 +475,18 @@ struct RowSet {
   const int *center = nullptr;
 };
 
-template <class T, class Metric, class Weight, bool COMPACT = false>
-__global__ void __launch_bounds__(256)
+// KEEP: every row keeps its kKeptKeys smallest keys over all other
+// components, written to kept[position][0 ..] (ascending, kEmptyKey padded),
+// and its need is the last of them, so that later rounds can settle the row
+// from the list (compBest must be null: a component's best would cut the
+// search at the row's first key).
+constexpr int kKeptKeys = 4;
+
+// The keeping walk runs the first round only; three waves per SIMD hide its
+// waits better than two despite 5 spilled registers (C3's four Borůvka walks:
+// 298 ms with two waves, 265 ms with three).
+template <class T, class Metric, class Weight, bool COMPACT = false, bool KEEP = false>
+__global__ void __launch_bounds__(256, (KEEP ? 3 : 1))
     nearestOtherWalkKernel(const float *__restrict__ X, int n, int d, Metric metric,
                            Weight weight, const int *__restrict__ comp,
                            const int *__restrict__ orig, Boxes boxes,
@@ -484,8 +494,10 @@ __global__ void __launch_bounds__(256)
                            unsigned long long *__restrict__ nearest,
                            unsigned long long *tilesComputed, int rowTileStride,
                            int computeSteps, const int *__restrict__ skipComp,
-                           const unsigned char *__restrict__ closedRows, RowSet rows) {
+                           const unsigned char *__restrict__ closedRows, RowSet rows,
+                           unsigned long long *__restrict__ kept) {
   constexpr int TR = T::TR;
+  __shared__ unsigned long long rowTop[KEEP ? T::DIM * kKeptKeys : 1];
   // Rows: the full order's tiles, or (COMPACT) the gathered rows, whose
   // position in the full order indexes comp, the weight values, closedRows
   // and nearest.
@@ -523,6 +535,9 @@ __global__ void __launch_bounds__(256)
     return comp[p] != skip && !(closedRows && closedRows[p]);
   };
   int anyOpen = 0;
+  if (KEEP)
+    for (int i = threadIdx.x; i < T::DIM * kKeptKeys; i += T::THREADS)
+      rowTop[i] = kEmptyKey;
   for (int i = threadIdx.x; i < T::DIM; i += T::THREADS) {
     rowBest[i] = kEmptyKey;
     const bool open = isOpen(row0 + i);
@@ -574,7 +589,8 @@ __global__ void __launch_bounds__(256)
   auto newBound = [&]() {
     float need = 0.0f;
     if (open0) {
-      const unsigned long long k = rowBest[threadIdx.x];
+      const unsigned long long k =
+          KEEP ? rowTop[threadIdx.x * kKeptKeys + kKeptKeys - 1] : rowBest[threadIdx.x];
       const float w = k == kEmptyKey ? INFINITY : keyWeight(k);
       if (compBest && w < cap0) {
         atomicMin(&compBest[comp0], bitsOf(w));
@@ -584,7 +600,8 @@ __global__ void __launch_bounds__(256)
       rowNeed[threadIdx.x] = need;
     }
     if (open1) {
-      const unsigned long long k = rowBest[threadIdx.x + 64];
+      const unsigned long long k = KEEP ? rowTop[(threadIdx.x + 64) * kKeptKeys + kKeptKeys - 1]
+                                        : rowBest[threadIdx.x + 64];
       const float w = k == kEmptyKey ? INFINITY : keyWeight(k);
       if (compBest && w < cap1) {
         atomicMin(&compBest[comp1], bitsOf(w));
@@ -653,6 +670,65 @@ __global__ void __launch_bounds__(256)
         colVal[c] = sColVal[i];
         colIdx[c] = sColIdx[i];
       }
+      if constexpr (KEEP) {
+        const int lane = threadIdx.x & 63;
+        const unsigned long long groupMask = 0xffffull << (lane & 48);
+#pragma unroll
+        for (int r = 0; r < TR; ++r) {
+          const int i = T::rowInTile(r);
+          unsigned long long *top = rowTop + i * kKeptKeys;
+          // Pairs that can enter the row's list (a stale last key only adds
+          // pairs that the insertion then rejects).
+          const float need = rowNeed[i];
+          const float limit = need < 0.0f ? -INFINITY
+                              : kFilter   ? metric.accBound(weight.distBound(need))
+                                          : INFINITY;
+          const unsigned long long last = top[kKeptKeys - 1];
+          unsigned int pending = 0;
+#pragma unroll
+          for (int c = 0; c < TR; ++c) {
+            const float acc = w[r][c];
+            if (!(acc > limit) && colComp[c] != rowComp[r] && rowComp[r] >= 0 && colComp[c] >= 0) {
+              const float ww = weight(kFilter ? metric.finish(acc) : acc, rowVal[r], colVal[c]);
+              w[r][c] = ww;
+              if (ww != INFINITY && packKey(ww, colIdx[c]) < last)
+                pending |= 1u << c;
+            }
+          }
+          // One insertion per row group at a time; the lanes stay converged.
+          while (true) {
+            const unsigned long long work = __ballot(pending != 0);
+            if (work == 0)
+              break;
+            const unsigned long long mine = work & groupMask;
+            if (mine != 0 && lane == __ffsll((long long)mine) - 1) {
+              const int c = __ffs(pending) - 1;
+              pending &= pending - 1;
+              float wv = 0.0f;
+              unsigned int idx = 0;
+#pragma unroll
+              for (int cc = 0; cc < TR; ++cc)
+                if (cc == c) {
+                  wv = w[r][cc];
+                  idx = colIdx[cc];
+                }
+              unsigned long long key = packKey(wv, idx);
+              if (key < top[kKeptKeys - 1]) {
+#pragma unroll
+                for (int q = 0; q < kKeptKeys; ++q) {
+                  const unsigned long long cur = top[q];
+                  top[q] = key < cur ? key : cur;
+                  key = key < cur ? cur : key;
+                }
+                tileImproved = 1;
+              }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          }
+        }
+      } else {
 #pragma unroll
       for (int r = 0; r < TR; ++r) {
         unsigned long long best = kEmptyKey;
@@ -691,6 +767,7 @@ __global__ void __launch_bounds__(256)
           }
         }
       }
+      }
       __syncthreads();
       // The needs and the bound change only with a row's best: a tile that
       // improved none skips the update and its barrier. Every wave reads the
@@ -709,6 +786,15 @@ __global__ void __launch_bounds__(256)
       }
     }
     __syncthreads(); // lowerBound is rewritten by the next batch
+  }
+  if (KEEP) {
+    for (int e = threadIdx.x; e < T::DIM * kKeptKeys; e += T::THREADS) {
+      const int i = e / kKeptKeys;
+      if (row0 + i < nRows)
+        kept[(size_t)posOf(row0 + i) * kKeptKeys + e % kKeptKeys] = rowTop[e];
+    }
+    for (int i = threadIdx.x; i < T::DIM; i += T::THREADS)
+      rowBest[i] = rowTop[i * kKeptKeys];
   }
   for (int i = threadIdx.x; i < T::DIM; i += T::THREADS)
     if (row0 + i < nRows && rowBest[i] != kEmptyKey)
@@ -982,7 +1068,8 @@ template <class T, class Metric, class Weight>
 int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &weight,
             const int *comp, const int *orig, unsigned int *compBest,
             unsigned long long *nearest, double *share, PassStats *stats, int stride,
-            const int *skipComp, const unsigned char *closedRows) {
+            const int *skipComp, const unsigned char *closedRows,
+            unsigned long long *kept = nullptr) {
   const int tiles = (n + T::DIM - 1) / T::DIM;
   DeviceBoxes boxes;
   DevArray<unsigned long long> counter;
@@ -998,11 +1085,23 @@ int runWalk(const float *X, int n, int d, const Metric &metric, const Weight &we
   const int sampled = (tiles + stride - 1) / stride;
   {
     PhaseTimer timer(kPhaseDistanceTiles);
-    hipLaunchKernelGGL((nearestOtherWalkKernel<T, Metric, Weight>), dim3(sampled), dim3(256),
-                       2 * d * sizeof(float), stream(), X, n, d, metric, weight, comp, orig,
-                       boxes.view, compBest, nearest, counter.get(), stride,
-                       stride > 1 ? kEstimateSteps : 0x7fffffff, skipComp, closedRows,
-                       RowSet{});
+    bool launched = false;
+    if constexpr (Weight::kHasValue) {
+      if (kept) {
+        hipLaunchKernelGGL((nearestOtherWalkKernel<T, Metric, Weight, false, true>),
+                           dim3(sampled), dim3(256), 2 * d * sizeof(float), stream(), X, n, d,
+                           metric, weight, comp, orig, boxes.view, nullptr, nearest,
+                           counter.get(), stride, stride > 1 ? kEstimateSteps : 0x7fffffff,
+                           skipComp, closedRows, RowSet{}, kept);
+        launched = true;
+      }
+    }
+    if (!launched)
+      hipLaunchKernelGGL((nearestOtherWalkKernel<T, Metric, Weight>), dim3(sampled), dim3(256),
+                         2 * d * sizeof(float), stream(), X, n, d, metric, weight, comp, orig,
+                         boxes.view, compBest, nearest, counter.get(), stride,
+                         stride > 1 ? kEstimateSteps : 0x7fffffff, skipComp, closedRows,
+                         RowSet{}, nullptr);
     s((int)hipGetLastError());
   }
   unsigned long long computed = 0;
@@ -1111,7 +1210,7 @@ int runWalkOpen(const float *X, int n, int d, const Metric &metric, const Weight
     hipLaunchKernelGGL((nearestOtherWalkKernel<T, Metric, Weight, true>), dim3(rowTiles),
                        dim3(256), 2 * d * sizeof(float), stream(), X, n, d, metric, weight, comp,
                        orig, colBoxes.view, compBest, nearest, counter.get(), 1, 0x7fffffff,
-                       skipComp, closedRows, rows);
+                       skipComp, closedRows, rows, nullptr);
     s((int)hipGetLastError());
   }
   unsigned long long computed = 0;
@@ -1131,9 +1230,11 @@ int runWalkOpen(const float *X, int n, int d, const Metric &metric, const Weight
 int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, const int *comp,
                      const int *orig, unsigned int *compBest, unsigned long long *nearest,
                      double *share, PassStats *stats, int stride, const int *skipComp,
-                     const unsigned char *closedRows) {
+                     const unsigned char *closedRows, unsigned long long *kept) {
   if (!metricHasBoxBound(m.id))
     return kErrInvalidMetric;
+  if (kept && (w.kind != kWeightMReach || closedRows || stride != 1))
+    return kErrInvalidAlgorithm;
   if (w.kind == kWeightWard) {
     if (m.id != kSqEuclidean)
       return kErrInvalidMetric;
@@ -1159,11 +1260,53 @@ int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, c
                                     share, stats, skipComp, closedRows);
           }
           return runWalk<T>(X, n, d, metric, weight, comp, orig, compBest, nearest, share,
-                            stats, stride, skipComp, closedRows);
+                            stats, stride, skipComp, closedRows, kept);
         });
       });
     }
   });
+}
+
+namespace {
+
+__global__ void settleFromKeptKernel(const unsigned long long *__restrict__ kept, int n,
+                                     const int *__restrict__ comp, const int *__restrict__ posOf,
+                                     const int *__restrict__ skipComp,
+                                     unsigned long long *__restrict__ nearest,
+                                     unsigned int *__restrict__ compBest,
+                                     unsigned char *__restrict__ closed) {
+  const int skip = skipComp ? *skipComp : -1;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    const int c = comp[p];
+    bool settled = false;
+    if (c != skip) {
+#pragma unroll
+      for (int q = 0; q < kKeptKeys && !settled; ++q) {
+        const unsigned long long key = kept[(size_t)p * kKeptKeys + q];
+        if (key == kEmptyKey)
+          break;
+        const int j = posOf ? posOf[keyIndex(key)] : (int)keyIndex(key);
+        if (comp[j] != c) {
+          nearest[p] = key;
+          if (compBest)
+            atomicMin(&compBest[c], (unsigned int)(key >> 32));
+          settled = true;
+        }
+      }
+    }
+    closed[p] = settled ? 1 : 0;
+  }
+}
+
+} // namespace
+
+int settleFromKept(const unsigned long long *kept, int n, const int *comp, const int *posOf,
+                   const int *skipComp, unsigned long long *nearest, unsigned int *compBest,
+                   unsigned char *closed) {
+  const int blocks = std::max(1, std::min(4 * numCUs(), (n + 255) / 256));
+  hipLaunchKernelGGL(settleFromKeptKernel, dim3(blocks), dim3(256), 0, stream(), kept, n, comp,
+                     posOf, skipComp, nearest, compBest, closed);
+  return (int)hipGetLastError();
 }
 
 int seedComponentBest(const unsigned long long *nearest, const int *comp, const int *posOf,

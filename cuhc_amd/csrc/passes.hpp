@@ -77,10 +77,26 @@ struct PassStats {
 // skipComp (nullable, device): rows of that component (-1: none) search
 // nothing and keep their nearest[] entries; so do rows p with closedRows[p]
 // set (nullable, device), whose exact answers are already in nearest[].
+// kept (nullable, device, n x 4 keys by position; mutual reachability, no
+// closed rows, no sampling): every row also keeps its 4 smallest keys over
+// the other components (ascending, empty keys ~0 padded), searched without
+// the component bounds, for settling rows in later rounds.
 int nearestOtherWalk(const float *X, int n, int d, MetricSpec m, WeightSpec w, const int *comp,
                      const int *orig, unsigned int *compBest, unsigned long long *nearest,
                      double *share, PassStats *stats, int rowTileStride = 1,
-                     const int *skipComp = nullptr, const unsigned char *closedRows = nullptr);
+                     const int *skipComp = nullptr, const unsigned char *closedRows = nullptr,
+                     unsigned long long *kept = nullptr);
+
+// Rows settled from the 4 keys each row kept in an earlier round (kept, by
+// position; keys carry original indices, posOf maps them to positions, null:
+// identity): a row whose list holds a point outside its component now takes
+// the first such key as its exact nearest (the list holds the row's smallest
+// keys over all other points, so every point outside the component that it
+// misses weighs more), lowers its component's best known weight and is
+// closed; the others, and rows of skipComp, stay open.
+int settleFromKept(const unsigned long long *kept, int n, const int *comp, const int *posOf,
+                   const int *skipComp, unsigned long long *nearest, unsigned int *compBest,
+                   unsigned char *closed);
 
 // *majority (device) = the component holding more than half of the n points,
 // or -1. sizes: numComp ints of scratch.
