@@ -105,6 +105,33 @@ def test_end_to_end_with_ties_against_sklearn(metric):
     assert adjusted_rand_score(ref.labels_, ours.labels_) >= 0.98
 
 
+@pytest.mark.parametrize("k, alpha", [(3, 1.0), (8, 1.0), (8, 0.6)])
+def test_pruned_ties_identical(k, alpha):
+    # A lattice (every point has several neighbours at exactly the same
+    # distance) plus duplicated points, shuffled: mutual-reachability weights
+    # tie at the core distances, so the four keys each row keeps in the first
+    # round, and the rows later rounds settle from them, meet ties broken by
+    # original index. Pruned runs (kept keys, open-row walks) against the
+    # unpruned pass and the oracle.
+    rng = np.random.default_rng(70 + k)
+    g = np.stack(np.meshgrid(np.arange(14), np.arange(14), np.arange(8)), axis=-1).reshape(-1, 3)
+    dup = np.repeat(rng.uniform(0, 14, size=(120, 3)), 3, axis=0)
+    X = np.concatenate([g, dup]).astype(np.float32)[rng.permutation(len(g) + len(dup))]
+    print(f"[hdbscan] pruned ties n={len(X)} k={k} alpha={alpha}", flush=True)
+    out = {}
+    for mode in ["always", "never"]:
+        _lib.set_pruning(mode)
+        try:
+            out[mode] = hdbscan_tree(X, k, alpha=alpha)
+        finally:
+            _lib.set_pruning("auto")
+    np.testing.assert_array_equal(out["always"][1], out["never"][1])
+    np.testing.assert_array_equal(out["always"][0], out["never"][0])
+    Zo, co = oracle.hdbscan_tree(X, k, alpha=alpha)
+    np.testing.assert_array_equal(out["never"][1], co)
+    np.testing.assert_array_equal(out["never"][0], Zo)
+
+
 def test_duplicates_and_single_point():
     X = np.repeat(np.random.default_rng(3).normal(size=(50, 3)), 4, axis=0).astype(np.float32)
     np.testing.assert_array_equal(cuhc.HDBSCAN(min_cluster_size=5, min_samples=4)
