@@ -135,18 +135,40 @@ __global__ void toHalfKernel(const float *X, int n, int d, int DP, const float *
                              _Float16 *xh, float *norm, float *R, float *rho,
                              unsigned int *maxR, unsigned int *maxRho) {
   float lr = 0.0f, lrho = 0.0f;
+  // A thread converts one point, eight features at a time: the row is read in
+  // 16-byte loads when X is 16-byte aligned and d is a multiple of 4 (every
+  // row then is), and each eight halves leave in one 16-byte store.
+  const bool vec4 = (d & 3) == 0 && ((size_t)X & 15) == 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     double nn = 0.0, err = 0.0;
-    for (int k = 0; k < DP; ++k) {
-      _Float16 h = (_Float16)0.0f;
-      if (k < d) {
-        const double xs = ((double)X[(size_t)i * d + k] - (double)mu[k]) * scale;
-        h = (_Float16)(float)xs;
-        const double hd = (double)(float)h;
-        nn += hd * hd;
-        err += (hd - xs) * (hd - xs);
+    const float *row = X + (size_t)i * d;
+    for (int k0 = 0; k0 < DP; k0 += 8) {
+      float v[8];
+      if (vec4 && k0 + 8 <= d) {
+        const float4 a = *reinterpret_cast<const float4 *>(row + k0);
+        const float4 b = *reinterpret_cast<const float4 *>(row + k0 + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          v[q] = k0 + q < d ? row[k0 + q] : 0.0f;
       }
-      xh[(size_t)i * DP + k] = h;
+      half8 h8;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int k = k0 + q;
+        _Float16 h = (_Float16)0.0f;
+        if (k < d) {
+          const double xs = ((double)v[q] - (double)mu[k]) * scale;
+          h = (_Float16)(float)xs;
+          const double hd = (double)(float)h;
+          nn += hd * hd;
+          err += (hd - xs) * (hd - xs);
+        }
+        h8[q] = h;
+      }
+      *reinterpret_cast<half8 *>(xh + (size_t)i * DP + k0) = h8;
     }
     const float nf = (float)nn;
     norm[i] = nf;
