@@ -42,7 +42,7 @@
 // row's component (the entries were the nearest outside a subset of the new
 // component, so the first such entry is exact); only rows whose whole list
 // joined their component are searched again. Rows whose list overflows are
-// rechecked in full, each by as many waves as 4,096-column chunks. This
+// rechecked in full, eight rows sharing each read of a 4,096-column chunk. This
 // translation unit is compiled with -fno-honor-nans (inputs are validated
 // finite), which lets fminf map to v_min/v_min3 without canonicalisation.
 #include "mst.hpp"
@@ -737,7 +737,7 @@ struct RecheckArgs {
   int *overflowCount; // zero before the recheck
 };
 
-// Columns one wave scans for a row whose list overflowed.
+// Columns one workgroup scans for the rows whose lists overflowed.
 constexpr int kOverflowWaveCols = 4096;
 
 constexpr int kStatStripes = 64;
@@ -898,49 +898,95 @@ __global__ void __launch_bounds__(256) recheckKernel(RecheckArgs a) {
   }
 }
 
-// Rows whose list overflowed: gridDim.y rows (overflowRows[first ..]), each
-// scanned by gridDim.x workgroups of four waves, wave w taking columns
-// [w * kOverflowWaveCols, (w + 1) * kOverflowWaveCols) and leaving its kTopK
-// smallest keys in partial[(row * waves + w) * kTopK ..].
-__global__ void __launch_bounds__(256) overflowScanKernel(RecheckArgs a, int first,
+// Rows whose list overflowed, kOverflowRowsPerGroup per workgroup (two per
+// wave) in the y dimension: workgroup x scans columns [x * kOverflowWaveCols,
+// (x + 1) * kOverflowWaveCols), staged 64 at a time in LDS so that every
+// column is read once for all of the workgroup's rows, and leaves each row's
+// kTopK smallest keys in partial[(row * gridDim.x + x) * kTopK ..] (row: the
+// row's place in this launch). The fold is recheckColumn's, feature by
+// feature, on the same values.
+constexpr int kOverflowRowsPerGroup = 8;
+__global__ void __launch_bounds__(256) overflowScanKernel(RecheckArgs a, int first, int rowsNow,
                                                          unsigned long long *partial) {
-  __shared__ float rowX[64];
+  __shared__ float rowX[kOverflowRowsPerGroup][64];
+  __shared__ float colX[64][65]; // [feature][column], padded: lane c reads column c
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int o = blockIdx.y, r = a.overflowRows[first + o];
-  const int pos = a.rows[3 * r], cs = a.rows[3 * r + 1], ce = a.rows[3 * r + 2];
-  for (int k = threadIdx.x; k < a.d; k += 256)
-    rowX[k] = a.xp[(size_t)pos * a.d + k];
-  __syncthreads();
-  const int waves = gridDim.x * 4, w = blockIdx.x * 4 + wave;
-  const long long col0 = (long long)w * kOverflowWaveCols;
-  const int end = (int)min((long long)a.n, col0 + kOverflowWaveCols);
-  unsigned long long best[kTopK];
+  const int local0 = blockIdx.y * kOverflowRowsPerGroup; // this launch's rows
+  const int rows = min(kOverflowRowsPerGroup, rowsNow - local0);
+  const int d = a.d;
+  for (int e = threadIdx.x; e < kOverflowRowsPerGroup * 64; e += 256) {
+    const int rr = e >> 6, k = e & 63;
+    if (rr < rows && k < d)
+      rowX[rr][k] = a.xp[(size_t)a.rows[3 * a.overflowRows[first + local0 + rr]] * d + k];
+  }
+  // This wave's two rows.
+  int cs[2], ce[2];
+  bool valid[2];
 #pragma unroll
-  for (int q = 0; q < kTopK; ++q)
-    best[q] = kEmptyKey;
-  for (long long col = col0 + lane; col < end; col += 64)
-    insertKey(best, recheckColumn(a, rowX, (int)col, cs, ce));
-  unsigned long long top[kTopK];
-  waveTopK(best, top);
-  if (lane < kTopK)
-    partial[((size_t)o * waves + w) * kTopK + lane] = lane == 0   ? top[0]
-                                                      : lane == 1 ? top[1]
-                                                      : lane == 2 ? top[2]
-                                                                  : top[3];
+  for (int q = 0; q < 2; ++q) {
+    const int rr = 2 * wave + q;
+    valid[q] = rr < rows;
+    const int r = valid[q] ? a.overflowRows[first + local0 + rr] : 0;
+    cs[q] = valid[q] ? a.rows[3 * r + 1] : 0;
+    ce[q] = valid[q] ? a.rows[3 * r + 2] : 0;
+  }
+  unsigned long long best[2][kTopK];
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int t = 0; t < kTopK; ++t)
+      best[q][t] = kEmptyKey;
+  const long long c0 = (long long)blockIdx.x * kOverflowWaveCols;
+  const int c1 = (int)min((long long)a.n, c0 + kOverflowWaveCols);
+  for (int s0 = (int)c0; s0 < c1; s0 += 64) {
+    __syncthreads(); // rowX written, or the previous stage consumed
+    for (int e = threadIdx.x; e < 64 * d; e += 256) {
+      const int c = e / d, k = e - c * d;
+      if (s0 + c < c1)
+        colX[k][c] = a.xp[(size_t)(s0 + c) * d + k];
+    }
+    __syncthreads();
+    const int col = s0 + lane;
+    if (col < c1) {
+      const int j = a.perm[col];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (!valid[q] || (col >= cs[q] && col < ce[q]))
+          continue;
+        const float *xi = rowX[2 * wave + q];
+        float acc = 0.0f;
+        for (int k = 0; k < d; ++k) {
+          const float diff = xi[k] - colX[k][lane];
+          acc = fmaf(diff, diff, acc);
+        }
+        const float w = a.sqeuclidean ? acc : sqrtf(acc);
+        if (w != INFINITY)
+          insertKey(best[q], packKey(w, (unsigned)j));
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    unsigned long long top[kTopK];
+    waveTopK(best[q], top);
+    if (valid[q] && lane < kTopK)
+      partial[((size_t)(local0 + 2 * wave + q) * gridDim.x + blockIdx.x) * kTopK + lane] =
+          lane == 0 ? top[0] : lane == 1 ? top[1] : lane == 2 ? top[2] : top[3];
+  }
 }
 
-// One wave per overflowed row: the kTopK smallest of its waves' keys.
+// One wave per overflowed row: the kTopK smallest of its column chunks' keys.
 __global__ void __launch_bounds__(64) overflowMergeKernel(RecheckArgs a, int first,
                                                          const unsigned long long *partial,
-                                                         int waves) {
+                                                         int chunks) {
   const int lane = threadIdx.x, o = blockIdx.x, r = a.overflowRows[first + o];
   const int pos = a.rows[3 * r];
   unsigned long long best[kTopK];
 #pragma unroll
   for (int q = 0; q < kTopK; ++q)
     best[q] = kEmptyKey;
-  const unsigned long long *keys = partial + (size_t)o * waves * kTopK;
-  for (int i = lane; i < waves * kTopK; i += 64)
+  const unsigned long long *keys = partial + (size_t)o * chunks * kTopK;
+  for (int i = lane; i < chunks * kTopK; i += 64)
     insertKey(best, keys[i]);
   unsigned long long top[kTopK];
   waveTopK(best, top);
@@ -1374,19 +1420,19 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   s(download(&overflowed, mp->overflowCount, sizeof(int)));
   if (s.ok() && overflowed > 0) {
     // Rows whose lists overflowed, at most 1,024 per batch and at most 32 MiB
-    // of partial keys: every column, kOverflowWaveCols per wave.
+    // of partial keys: every column, kOverflowWaveCols per workgroup.
     PhaseTimer timer(kPhaseRecheck);
-    const int groups = (int)((((long long)n + kOverflowWaveCols - 1) / kOverflowWaveCols + 3) / 4);
-    const int waves = groups * 4;
-    const int batch = std::max(1, std::min({overflowed, 1024, (32 << 20) / (waves * kTopK * 8)}));
+    const int groups = (int)(((long long)n + kOverflowWaveCols - 1) / kOverflowWaveCols);
+    const int batch = std::max(1, std::min({overflowed, 1024, (32 << 20) / (groups * kTopK * 8)}));
     DevArray<unsigned long long> partial;
-    s(partial.alloc((size_t)batch * waves * kTopK));
+    s(partial.alloc((size_t)batch * groups * kTopK));
     for (int first = 0; s.ok() && first < overflowed; first += batch) {
       const int rowsNow = std::min(batch, overflowed - first);
-      hipLaunchKernelGGL(overflowScanKernel, dim3(groups, rowsNow), dim3(256), 0, stream(), ra,
-                         first, partial.get());
+      const int rowGroups = (rowsNow + kOverflowRowsPerGroup - 1) / kOverflowRowsPerGroup;
+      hipLaunchKernelGGL(overflowScanKernel, dim3(groups, rowGroups), dim3(256), 0, stream(), ra,
+                         first, rowsNow, partial.get());
       hipLaunchKernelGGL(overflowMergeKernel, dim3(rowsNow), dim3(64), 0, stream(), ra, first,
-                         partial.get(), waves);
+                         partial.get(), groups);
       s((int)hipGetLastError());
     }
   }
