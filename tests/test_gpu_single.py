@@ -51,14 +51,16 @@ def test_mixed_on_clustered_and_offset_data():
     np.testing.assert_array_equal(run(Y, precision="mixed"), run(Y, precision="fp32"))
 
 
-def test_mixed_overflowing_lists():
+@pytest.mark.parametrize("d", [16, 18])
+def test_mixed_overflowing_lists(d):
     # 150 copies of each of 200 points, shuffled: every row has 149 columns at
     # weight 0 spread over about as many half-blocks, more than a candidate
-    # list holds, so its rows are rechecked by the chunked full scan.
+    # list holds, so its rows are rechecked by the chunked full scan (d = 18:
+    # rows not a multiple of 4 floats, the scalar load paths).
     rng = np.random.default_rng(8)
-    base = rng.normal(size=(200, 16)).astype(np.float32)
+    base = rng.normal(size=(200, d)).astype(np.float32)
     X = np.repeat(base, 150, axis=0)[rng.permutation(30_000)]
-    print("[single] overflowing lists n=30000 d=16", flush=True)
+    print(f"[single] overflowing lists n=30000 d={d}", flush=True)
     _lib.mixed_stats()
     Z = run(X, precision="mixed")
     stats = _lib.mixed_stats()
