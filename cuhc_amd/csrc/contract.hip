@@ -229,14 +229,19 @@ int contractRound(Contraction &ct, const unsigned long long *nearest, int *comp,
   *added = count - ct.edges.count;
   ct.edges.count = count;
   // Pointer jumping until every component points at its root (log depth;
-  // 40 doublings cover any depth below 2^31).
+  // 40 doublings cover any depth below 2^31). The jumps run in batches of
+  // four with one read of the flag per batch: it is cleared before each jump,
+  // so it tells whether the batch's last jump still moved a pointer.
+  constexpr int kJumpsPerRead = 4;
   bool converged = false;
-  for (int iter = 0; iter < 40 && !converged; ++iter) {
-    int changed = 0;
-    s(fill(ct.changed, 0, sizeof(int)));
-    hipLaunchKernelGGL(jumpKernel, gridFor(numComp), dim3(256), 0, stream(), ct.root.get(),
-                       numComp, ct.changed.get());
+  for (int iter = 0; iter < 40 && !converged; iter += kJumpsPerRead) {
+    for (int j = 0; j < kJumpsPerRead; ++j) {
+      s(fill(ct.changed, 0, sizeof(int)));
+      hipLaunchKernelGGL(jumpKernel, gridFor(numComp), dim3(256), 0, stream(), ct.root.get(),
+                         numComp, ct.changed.get());
+    }
     s((int)hipGetLastError());
+    int changed = 0;
     s(download(&changed, ct.changed, sizeof(int)));
     if (!s.ok())
       return s.code;
