@@ -72,7 +72,7 @@ constexpr int kCandCap = 64; // list entries per (row, half)
 constexpr int kTopK = 4; // nearest points kept per row for the next round
 constexpr int kDryTilesFirstRound = 16;
 constexpr int kDryTilesLaterRounds = 4; // per column split
-constexpr int kMaxSplits = 128;         // column splits of a later round
+constexpr int kMaxSplits = 256;         // column splits of a later round
 
 std::atomic<int> gLayoutChecked{0}; // 0 unknown, 1 ok, -1 mismatch
 unsigned long long gStats[4];
@@ -1356,8 +1356,10 @@ int mixedPassRun(MixedPass *mp, const int *comp, int numComponents, bool sqeucli
   // the fewest splits whose workgroups fill the last wave of resident slots
   // (two per CU) at least 90%, else the best fill. Late rounds with few rows
   // would otherwise leave CUs idle (rounds 2-5 at 10^6 points: 6.6 ms with
-  // at most 16 splits, about 4.3 ms with 128), and a grid just above a
-  // multiple of the resident slots would run its last wave nearly empty.
+  // at most 16 splits, about 3.2 ms with 128 or 256; with 256 no list of
+  // those rounds overflowed, against about 20 with 128), and a grid just
+  // above a multiple of the resident slots would run its last wave nearly
+  // empty.
   const long long slots = 2LL * numCUs();
   const int maxSplits = std::max(1, std::min(kMaxSplits, mp->nPad / 2048));
   int splits = 1;
